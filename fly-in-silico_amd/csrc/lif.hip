@@ -1,0 +1,537 @@
+// libfis: MI355X (gfx950) kernels for the whole-brain LIF step.
+//
+// Semantics: Engine._core + Engine._poisson of lab/flybrain/sim.py:848-953
+// and the event SpMV of lab/flybrain/ev_spmv.cu:60-79.  Layout and launch
+// shapes are this library's own:
+//
+//  * state is trial-minor, idx = neuron * bstride + trial, so one wave of 64
+//    lanes is 64 trials of one neuron (bstride >= 64) or 64/bstride neurons
+//    with all their trials; a spike ballot is then directly the trial mask
+//    of an event, and the scatter reads a CSR row once for every trial in
+//    that mask;
+//  * the delay ring is int32 synapse counts, 20 slots: the update of step t
+//    reads and clears slot (t+1) mod 20, which holds the input computed from
+//    the spikes of step t-20 (reference: rec written at step u, read at
+//    u+19); the scatter of step t's spikes adds into the same slot.  Since
+//    an input needs 20 steps to arrive, W <= 20 steps can run back to back
+//    with v, g and the refractory counter in registers before one scatter;
+//  * integer atomics make the sum exact and order-free; wScale is applied
+//    once per read, after the sum (sim.py:906-918);
+//  * every float32 operation of the reference is kept as one rounding in
+//    the same order (build with -ffp-contract=off).
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "../../include/fis.h"
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+
+// ------------------------------------------------------------- Philox
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c.x);
+    const uint32_t lo0 = 0xD2511F53u * c.x;
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z);
+    const uint32_t lo1 = 0xCD9E8D57u * c.z;
+    c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+  }
+  return c;
+}
+
+__device__ __forceinline__ uint32_t pick(const U4& x, uint32_t j) {
+  return j == 0 ? x.x : (j == 1 ? x.y : (j == 2 ? x.z : x.w));
+}
+
+// Bernoulli(p) from one 32-bit word: u = (x >> 8) * 2^-24 (exact), u < p.
+__device__ __forceinline__ uint32_t bern(uint32_t x, float p) {
+  const float u = (float)(x >> 8) * (1.0f / 16777216.0f);
+  return u < p ? 1u : 0u;
+}
+
+// Kick bits of one neuron-trial for steps t0 .. t0+W-1 at a probability
+// that may change per step (p_of(t)).  Counter (t>>2, neuron, trial_id, 0),
+// key = 64-bit seed, word t & 3.
+template <class P>
+__device__ __forceinline__ uint32_t window_bits(int64_t t0, int W,
+                                                uint32_t neuron, uint32_t tid,
+                                                uint64_t seed, P p_of) {
+  const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  uint32_t bits = 0;
+  U4 x{0, 0, 0, 0};
+  for (int k = 0; k < W; ++k) {
+    const int64_t t = t0 + k;
+    if (k == 0 || (t & 3) == 0)
+      x = philox(U4{(uint32_t)(t >> 2), neuron, tid, 0u}, k0, k1);
+    const float p = p_of(t);
+    if (p > 0.0f) bits |= bern(pick(x, (uint32_t)(t & 3)), p) << k;
+  }
+  return bits;
+}
+
+__device__ __forceinline__ int64_t n_waves_of(const fis_sim& s) {
+  const int64_t nb = (int64_t)s.n * s.bstride;
+  return (nb + kWave - 1) / kWave;
+}
+
+// ------------------------------------------------------------- drive
+
+__global__ __launch_bounds__(kBlock) void k_drive_rates(fis_sim s,
+                                                        const float* rates,
+                                                        int W) {
+  const int64_t nb = (int64_t)s.n * s.bstride;
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= nb) return;
+  const int64_t i = idx / s.bstride;
+  const int b = (int)(idx - i * s.bstride);
+  uint32_t bits = 0;
+  if (b < s.batch) {
+    const float p = rates[(int64_t)b * s.n + i] * s.prob_scale;
+    if (p > 0.0f)
+      bits = window_bits(*s.clock, W, (uint32_t)i, s.trial_ids[b], s.seeds[b],
+                         [p](int64_t) { return p; });
+  }
+  s.sbits[idx] = bits;
+}
+
+__global__ __launch_bounds__(kBlock) void k_drive_table(
+    fis_sim s, const fis_drive_entry* entries, int n_entries,
+    const float* rate_hz, int n_groups, int n_seg, int seg_steps, int W) {
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n_entries) return;
+  const fis_drive_entry e = entries[j];
+  const int64_t i = (int64_t)e.idx / s.bstride;
+  const int b = (int)((int64_t)e.idx - i * s.bstride);
+  const float ps = s.prob_scale;
+  const uint32_t bits = window_bits(
+      *s.clock, W, (uint32_t)i, s.trial_ids[b], s.seeds[b], [&](int64_t t) {
+        int64_t seg = t / seg_steps;
+        if (seg >= n_seg) seg = n_seg - 1;
+        return rate_hz[seg * n_groups + e.group] * ps;
+      });
+  s.sbits[e.idx] = bits;
+}
+
+__global__ __launch_bounds__(kBlock) void k_drive_tape(fis_sim s,
+                                                       const int64_t* tape_ptr,
+                                                       const int32_t* tape_idx,
+                                                       int64_t n_tape_steps) {
+  // blockIdx.y = window step k
+  const int64_t t = *s.clock + blockIdx.y;
+  if (t >= n_tape_steps) return;
+  const int64_t beg = tape_ptr[t], end = tape_ptr[t + 1];
+  for (int64_t e = beg + (int64_t)blockIdx.x * kBlock + threadIdx.x; e < end;
+       e += (int64_t)gridDim.x * kBlock)
+    atomicOr(&s.sbits[tape_idx[e]], 1u << blockIdx.y);
+}
+
+// ------------------------------------------------------------- update
+
+// Spike compaction of one step: the raster row (optional) and the events
+// of transmitted spikes, appended to this wave's segment.
+__device__ __forceinline__ void emit(const fis_sim& s, bool fired,
+                                     bool sends, int lane, int64_t idx,
+                                     int64_t wave, int64_t t, int slot,
+                                     fis_event* evs, int& nev) {
+  const int bs = s.bstride;
+  if (s.raster) {
+    const uint64_t all = __ballot(fired);
+    if (lane == 0) s.raster[(t % s.raster_len) * n_waves_of(s) + wave] = all;
+  }
+  const uint64_t bal = __ballot(sends);
+  if (!bal) return;
+  if (bs >= kWave) {
+    if (lane == 0) {
+      fis_event e;
+      e.pre = (int32_t)(idx / bs);
+      e.slot = (uint16_t)slot;
+      e.tb = (uint16_t)(idx % bs);
+      e.mask = bal;
+      evs[nev] = e;
+    }
+    nev += 1;
+  } else {
+    // 64/bs neurons per wave; the first lane of each neuron leads
+    const int j0 = lane - (lane % bs);
+    const uint64_t sub = (bal >> j0) & ((1ull << bs) - 1ull);
+    const bool lead = (lane % bs) == 0 && sub != 0;
+    const uint64_t lb = __ballot(lead);
+    if (lead) {
+      const int pos = __popcll(lb & ((1ull << lane) - 1ull));
+      fis_event e;
+      e.pre = (int32_t)(idx / bs);
+      e.slot = (uint16_t)slot;
+      e.tb = 0;
+      e.mask = sub;
+      evs[nev + pos] = e;
+    }
+    nev += __popcll(lb);
+  }
+}
+
+// W steps per neuron-trial.  WMAX is the unrolled bound (1 or 20).
+template <int WMAX>
+__global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
+  const int64_t nb = (int64_t)s.n * s.bstride;
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t wave = idx >> 6;
+  const bool act = idx < nb;
+  const int64_t t0 = *s.clock;
+  const int bs = s.bstride;
+
+  float v = 0.0f, g = 0.0f, vh = 0.0f;
+  uint32_t st = 0, fl = FIS_FIRE_OFF, sb = 0;
+  if (act) {
+    v = s.v[idx];
+    g = s.g[idx];
+    st = s.st[idx];
+    fl = s.fl[idx];
+    sb = s.sbits[idx];
+    if ((fl & FIS_HOLD) && s.vhold) vh = s.vhold[idx];
+  }
+  if (act && sb) s.sbits[idx] = 0u;
+
+  // masks as the reference's float factors (exactly 0 or 1)
+  const float m_fire = (fl & FIS_FIRE_OFF) ? 0.0f : 1.0f;
+  const float m_in = (fl & FIS_IN_OFF) ? 0.0f : 1.0f;
+  const float m_hold = (fl & FIS_HOLD) ? 0.0f : 1.0f;
+  const float v_hold = (fl & FIS_HOLD) ? vh : 0.0f;
+  const int R = (fl & FIS_NOREFRAC) ? 0 : s.refrac_steps;
+  const bool transmit = act && !(fl & FIS_OUT_OFF);
+  // stim = w_scale * (bern * gain), sim.py:859-861
+  const float kick1 = s.w_scale * (1.0f * s.gain);
+  const float kick0 = s.w_scale * (0.0f * s.gain);
+
+  int r = (int)(st & FIS_ST_RMAX);
+  float sp = (st & FIS_ST_SPIKE) ? 1.0f : 0.0f;
+
+  // the W ring reads are independent of the recurrence: issue them first
+  int32_t* rp = s.ring + idx;
+  const int slot0 = (int)((t0 + 1) % FIS_RING_LEN);
+  int32_t in[WMAX];
+#pragma unroll
+  for (int k = 0; k < WMAX; ++k) {
+    int slot = slot0 + k;
+    slot -= (slot >= FIS_RING_LEN) ? FIS_RING_LEN : 0;
+    in[k] = (k < W && act) ? rp[(int64_t)slot * nb] : 0;
+  }
+
+  fis_event* evs = s.events + wave * s.ev_cap;
+  int nev = 0;  // wave-uniform
+  uint32_t nspk = 0;
+
+#pragma unroll
+  for (int k = 0; k < WMAX; ++k) {
+    if (k < W) {
+      int slot = slot0 + k;
+      slot -= (slot >= FIS_RING_LEN) ? FIS_RING_LEN : 0;
+      const int32_t I = in[k];
+      if (act && I != 0) rp[(int64_t)slot * nb] = 0;
+
+      // refractory counter: r = (r + 1) * (1 - s), saturating
+      r = (sp != 0.0f) ? 0 : (r < FIS_ST_RMAX ? r + 1 : FIS_ST_RMAX);
+      const float gate = (r >= R) ? 1.0f : 0.0f;
+
+      // conductance with the delayed input (sim.py:927-929)
+      const float rec = s.w_scale * (float)I;
+      const float del = rec * m_in;
+      const float gd = g * s.a_syn;
+      const float dg = del * gate;
+      const float g_new = gd + dg;
+
+      // membrane with the OLD conductance (sim.py:937-939)
+      const float stim = ((sb >> k) & 1u) ? kick1 : kick0;
+      float vv = v + stim;
+      const float t1 = vv - s.v_rest;
+      const float t2 = g - t1;
+      const float t3 = s.a_mem * t2;
+      vv = vv + t3;
+      vv = vv * m_hold;  // hold before threshold (sim.py:944)
+      vv = vv + v_hold;
+      float spk = (vv > s.v_th) ? 1.0f : 0.0f;
+      spk = spk * m_fire;
+      const float t4 = vv - s.v_reset;  // reset by the masked spike
+      const float t5 = t4 * spk;
+      vv = vv - t5;
+      const float gs = g_new * spk;
+      g = g_new - gs;
+      v = vv;
+      sp = spk;
+
+      const bool fired = act && spk != 0.0f;
+      nspk += fired ? 1u : 0u;
+      emit(s, fired, fired && transmit, lane, idx, wave, t0 + k, slot, evs,
+           nev);
+    }
+  }
+
+  if (act) {
+    s.v[idx] = v;
+    s.g[idx] = g;
+    s.st[idx] = (uint8_t)((sp != 0.0f ? FIS_ST_SPIKE : 0u) | (uint32_t)r);
+    if (s.spikes_out) {
+      const int64_t i = idx / bs;
+      const int b = (int)(idx - i * bs);
+      if (b < s.batch) s.spikes_out[(int64_t)b * s.n + i] = sp;
+    }
+    if (nspk) {
+      if (s.counts) s.counts[idx] += nspk;
+      if (s.pop_counts) atomicAdd(&s.pop_counts[idx / bs], nspk);
+    }
+  }
+  if (lane == 0 && act) s.ev_count[wave] = nev;
+
+  // the last block to finish advances the clock for the next window
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const uint32_t prev = atomicAdd(s.done, 1u);
+    if (prev == gridDim.x - 1) {
+      *s.clock = t0 + W;
+      *s.done = 0u;
+      __threadfence();
+    }
+  }
+}
+
+// ------------------------------------------------------------- scatter
+
+// One wave per update-wave segment (grid-stride); each event's CSR row is
+// read once and added into every trial of its mask.
+__global__ __launch_bounds__(kBlock) void k_scatter(fis_sim s) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t nw = n_waves_of(s);
+  const int64_t nb = (int64_t)s.n * s.bstride;
+  const int bs = s.bstride;
+  const int64_t stride = (int64_t)gridDim.x * (kBlock / kWave);
+  for (int64_t w = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6; w < nw;
+       w += stride) {
+    const int cnt = __builtin_amdgcn_readfirstlane(s.ev_count[w]);
+    if (cnt == 0) continue;
+    const fis_event* evs = s.events + w * s.ev_cap;
+    for (int e = 0; e < cnt; ++e) {
+      const fis_event ev = evs[e];
+      const int64_t beg = s.offsets[ev.pre], end = s.offsets[ev.pre + 1];
+      int32_t* base = s.ring + (int64_t)ev.slot * nb + ev.tb;
+      for (int64_t k = beg + lane; k < end; k += kWave) {
+        const int32_t post = s.cols[k];
+        const int32_t val = s.vals[k];
+        int32_t* p = base + (int64_t)post * bs;
+        uint64_t m = ev.mask;
+        while (m) {
+          const int j = __ffsll((long long)m) - 1;
+          m &= m - 1ull;
+          __hip_atomic_fetch_add(p + j, val, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+    }
+  }
+}
+
+// Current spikes (st bit 7, transmitted) into one ring slot.
+__global__ __launch_bounds__(kBlock) void k_seed_ring(fis_sim s, int slot) {
+  const int64_t nb = (int64_t)s.n * s.bstride;
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  const bool hot = idx < nb && (s.st[idx] & FIS_ST_SPIKE) &&
+                   !(s.fl[idx] & FIS_OUT_OFF);
+  uint64_t bal = __ballot(hot);
+  const int64_t base_idx = idx - lane;
+  while (bal) {
+    const int j = __ffsll((long long)bal) - 1;
+    bal &= bal - 1ull;
+    const int64_t src = base_idx + j;
+    const int64_t pre = src / s.bstride;
+    const int b = (int)(src - pre * s.bstride);
+    int32_t* base = s.ring + (int64_t)slot * nb + b;
+    for (int64_t k = s.offsets[pre] + lane; k < s.offsets[pre + 1]; k += kWave)
+      __hip_atomic_fetch_add(base + (int64_t)s.cols[k] * s.bstride,
+                             (int32_t)s.vals[k], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// ------------------------------------------------------------- ev_spmv twin
+
+// out[b, post] += sum_pre W[post, pre] * sp[b, pre] * mo[b, pre].  Each lane
+// tests one presynaptic neuron; the wave then walks every hot row together.
+template <typename I>
+__global__ __launch_bounds__(kBlock) void k_ev_spmv(
+    const I* __restrict__ crow, const I* __restrict__ col,
+    const float* __restrict__ val, const float* __restrict__ sp,
+    const float* __restrict__ mo, float* __restrict__ out, int n) {
+  const int64_t off = (int64_t)blockIdx.y * n;
+  const int64_t pre = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  float x = 0.0f;
+  if (pre < n) x = sp[off + pre] * mo[off + pre];
+  uint64_t bal = __ballot(x != 0.0f);
+  const int64_t base_pre = pre - lane;
+  while (bal) {
+    const int j = __ffsll((long long)bal) - 1;
+    bal &= bal - 1ull;
+    const int64_t p = base_pre + j;
+    const float sj = __shfl(x, j);
+    const I a = crow[p], e = crow[p + 1];
+    for (I k = a + (I)lane; k < e; k += (I)kWave)
+      unsafeAtomicAdd(out + off + (int64_t)col[k], val[k] * sj);
+  }
+}
+
+bool valid_stride(int32_t bs) {
+  if (bs <= 0) return false;
+  if (bs < kWave) return (bs & (bs - 1)) == 0;
+  return bs % kWave == 0;
+}
+
+int check_sim(const fis_sim* s) {
+  if (!s || s->n <= 0 || s->batch <= 0 || !valid_stride(s->bstride) ||
+      s->batch > s->bstride || s->bstride > 65535 + 1)
+    return (int)hipErrorInvalidValue;
+  const int need = FIS_MAX_WINDOW * (s->bstride >= kWave ? 1 : kWave / s->bstride);
+  if (s->ev_cap < need) return (int)hipErrorInvalidValue;
+  if (!s->offsets || !s->cols || !s->vals || !s->v || !s->g || !s->st ||
+      !s->fl || !s->ring || !s->sbits || !s->events || !s->ev_count ||
+      !s->clock || !s->done || !s->seeds || !s->trial_ids)
+    return (int)hipErrorInvalidValue;
+  if (s->raster && s->raster_len <= 0) return (int)hipErrorInvalidValue;
+  return 0;
+}
+
+unsigned grid_for(int64_t items) {
+  return (unsigned)((items + kBlock - 1) / kBlock);
+}
+
+}  // namespace
+
+extern "C" {
+
+int fis_abi_version(void) { return FIS_ABI_VERSION; }
+
+int fis_ev_spmv_abi(void) { return 3; }
+
+int64_t fis_struct_layout(int what) {
+  switch (what) {
+    case 0: return (int64_t)sizeof(fis_sim);
+    case 1: return (int64_t)offsetof(fis_sim, offsets);
+    case 2: return (int64_t)offsetof(fis_sim, w_scale);
+    case 3: return (int64_t)offsetof(fis_sim, refrac_steps);
+    case 4: return (int64_t)offsetof(fis_sim, raster_len);
+    case 5: return (int64_t)sizeof(fis_event);
+    case 6: return (int64_t)sizeof(fis_drive_entry);
+    default: return -1;
+  }
+}
+
+int64_t fis_n_waves(int32_t n, int32_t bstride) {
+  const int64_t nb = (int64_t)n * bstride;
+  return (nb + kWave - 1) / kWave;
+}
+
+int fis_ev_spmv_i32(const int32_t* crow, const int32_t* col, const float* val,
+                    const float* sp, const float* mo, float* out, int n,
+                    int batch, int threads, void* stream) {
+  (void)threads;
+  if (n <= 0 || batch <= 0) return (int)hipErrorInvalidValue;
+  dim3 grid(grid_for(n), (unsigned)batch);
+  hipLaunchKernelGGL(k_ev_spmv<int32_t>, grid, dim3(kBlock), 0,
+                     (hipStream_t)stream, crow, col, val, sp, mo, out, n);
+  return (int)hipGetLastError();
+}
+
+int fis_ev_spmv_i64(const int64_t* crow, const int64_t* col, const float* val,
+                    const float* sp, const float* mo, float* out, int n,
+                    int batch, int threads, void* stream) {
+  (void)threads;
+  if (n <= 0 || batch <= 0) return (int)hipErrorInvalidValue;
+  dim3 grid(grid_for(n), (unsigned)batch);
+  hipLaunchKernelGGL(k_ev_spmv<int64_t>, grid, dim3(kBlock), 0,
+                     (hipStream_t)stream, crow, col, val, sp, mo, out, n);
+  return (int)hipGetLastError();
+}
+
+int fis_drive_rates(const fis_sim* s, const float* rates, int W,
+                    void* stream) {
+  if (int rc = check_sim(s)) return rc;
+  if (!rates || W < 1 || W > FIS_MAX_WINDOW) return (int)hipErrorInvalidValue;
+  const int64_t nb = (int64_t)s->n * s->bstride;
+  hipLaunchKernelGGL(k_drive_rates, dim3(grid_for(nb)), dim3(kBlock), 0,
+                     (hipStream_t)stream, *s, rates, W);
+  return (int)hipGetLastError();
+}
+
+int fis_drive_table(const fis_sim* s, const fis_drive_entry* entries,
+                    int n_entries, const float* rate_hz, int n_groups,
+                    int n_seg, int seg_steps, int W, void* stream) {
+  if (int rc = check_sim(s)) return rc;
+  if (W < 1 || W > FIS_MAX_WINDOW || n_entries < 0 || n_groups <= 0 ||
+      n_seg <= 0 || seg_steps <= 0 || !rate_hz ||
+      (n_entries > 0 && !entries))
+    return (int)hipErrorInvalidValue;
+  if (n_entries == 0) return 0;
+  hipLaunchKernelGGL(k_drive_table, dim3(grid_for(n_entries)), dim3(kBlock), 0,
+                     (hipStream_t)stream, *s, entries, n_entries, rate_hz,
+                     n_groups, n_seg, seg_steps, W);
+  return (int)hipGetLastError();
+}
+
+int fis_drive_tape(const fis_sim* s, const int64_t* tape_ptr,
+                   const int32_t* tape_idx, int64_t n_tape_steps, int W,
+                   void* stream) {
+  if (int rc = check_sim(s)) return rc;
+  if (W < 1 || W > FIS_MAX_WINDOW || !tape_ptr || n_tape_steps < 0)
+    return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_drive_tape, dim3(8, (unsigned)W), dim3(kBlock), 0,
+                     (hipStream_t)stream, *s, tape_ptr, tape_idx,
+                     n_tape_steps);
+  return (int)hipGetLastError();
+}
+
+int fis_update(const fis_sim* s, int W, void* stream) {
+  if (int rc = check_sim(s)) return rc;
+  if (W < 1 || W > FIS_MAX_WINDOW) return (int)hipErrorInvalidValue;
+  const int64_t nb = (int64_t)s->n * s->bstride;
+  if (W == 1)
+    hipLaunchKernelGGL(k_update<1>, dim3(grid_for(nb)), dim3(kBlock), 0,
+                       (hipStream_t)stream, *s, W);
+  else
+    hipLaunchKernelGGL(k_update<FIS_MAX_WINDOW>, dim3(grid_for(nb)),
+                       dim3(kBlock), 0, (hipStream_t)stream, *s, W);
+  return (int)hipGetLastError();
+}
+
+int fis_scatter(const fis_sim* s, void* stream) {
+  if (int rc = check_sim(s)) return rc;
+  const int64_t nw = fis_n_waves(s->n, s->bstride);
+  int64_t blocks = (nw + (kBlock / kWave) - 1) / (kBlock / kWave);
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_scatter, dim3((unsigned)blocks), dim3(kBlock), 0,
+                     (hipStream_t)stream, *s);
+  return (int)hipGetLastError();
+}
+
+int fis_seed_ring(const fis_sim* s, int slot, void* stream) {
+  if (int rc = check_sim(s)) return rc;
+  if (slot < 0 || slot >= FIS_RING_LEN) return (int)hipErrorInvalidValue;
+  const int64_t nb = (int64_t)s->n * s->bstride;
+  hipLaunchKernelGGL(k_seed_ring, dim3(grid_for(nb)), dim3(kBlock), 0,
+                     (hipStream_t)stream, *s, slot);
+  return (int)hipGetLastError();
+}
+
+}  // extern "C"

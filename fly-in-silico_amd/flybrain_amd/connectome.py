@@ -301,7 +301,9 @@ def synthetic(n, n_edges=None, *, mean_out=109.0, sigma=1.0, count_exp=2.0,
       to [1, n-1] and rescaled so the total is exactly ``n_edges``;
     * targets are distinct, never the neuron itself;
     * with ``n_groups > 1`` neurons fall into groups of log-normal size and a
-      fraction ``p_within`` of each neuron's edges stays in its own group;
+      fraction ``p_within`` of all edges stays inside the presynaptic
+      neuron's group (rows of groups too small for their share spill out,
+      the other rows keep slightly more inside to compensate);
       group membership is a seeded random permutation of neuron ids, so a
       group is not a contiguous id range;
     * synapse count ~ discrete power law P(k) ~ k^-count_exp on [1, count_max];
@@ -326,8 +328,23 @@ def synthetic(n, n_edges=None, *, mean_out=109.0, sigma=1.0, count_exp=2.0,
     perm = rng.permutation(n)
 
     gsz = size[gpos]
-    k_in = np.rint(deg * p_within).astype(np.int64)
-    k_in = np.minimum(k_in, gsz - 1)
+
+    def within(p):
+        return np.minimum(np.rint(deg * p).astype(np.int64), gsz - 1)
+
+    # rows of small groups cannot keep p_within of their edges inside; raise
+    # the share of the others so the graph as a whole has p_within inside
+    target = int(round(p_within * deg.sum()))
+    lo, hi = float(p_within), 1.0
+    if within(hi).sum() <= target:
+        lo = hi
+    for _ in range(40):
+        mid = 0.5 * (lo + hi)
+        if within(mid).sum() < target:
+            lo = mid
+        else:
+            hi = mid
+    k_in = within(lo)
     k_out = deg - k_in
     cap_out = n - gsz
     over = k_out > cap_out

@@ -1,0 +1,397 @@
+"""Device state of one batch of LIF trials and the launches that advance it.
+
+``DeviceNet`` holds the connectome (by presynaptic neuron) in HBM;
+``DeviceSim`` holds the state of ``batch`` trials in the trial-minor layout
+the kernels use (idx = neuron * bstride + trial) and enqueues the C-ABI
+entry points of libfis.so on the current torch stream.  torch is used for
+device memory and streams only; all arithmetic of the step is in the HIP
+kernels.
+
+Memory per neuron-trial: v, g (4 B each), st, fl (1 B each), sbits (4 B),
+ring 20 x 4 B -- about 94 B; at 138,639 neurons x 256 trials that is
+3.3 GB plus events, far inside one MI355X's 288 GB.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .params import derived
+
+
+def stride_for(batch: int) -> int:
+    """Trial stride: next power of two below 64, next multiple of 64 above."""
+    if batch < 1:
+        raise ValueError("batch must be >= 1")
+    if batch >= 64:
+        return (batch + 63) // 64 * 64
+    p = 1
+    while p < batch:
+        p *= 2
+    return p
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class DeviceNet:
+    """The synapse matrix by presynaptic neuron, resident in HBM."""
+
+    def __init__(self, conn, device="cuda"):
+        self.conn = conn
+        self.n = conn.n
+        self.device = torch.device(device)
+        self.offsets = torch.from_numpy(conn.offsets).to(self.device)
+        self.cols = torch.from_numpy(conn.cols).to(self.device)
+        self.vals = torch.from_numpy(conn.vals).to(self.device)
+
+    @property
+    def n_edges(self):
+        return self.conn.n_edges
+
+
+class DeviceSim:
+    """State of ``batch`` independent trials on one GPU."""
+
+    def __init__(self, net: DeviceNet, batch: int, params=None, seeds=None,
+                 trial_ids=None, no_refractory=None, spikes_out=False,
+                 counts=False, pop_counts=False, raster_len=0):
+        self.lib = L.lib()
+        if net.device.type != "cuda":
+            raise L.KernelUnavailable("the engine runs on a GPU only; got "
+                                      "device %r" % (net.device,))
+        self.net = net
+        self.c = derived(params)
+        self.n = n = net.n
+        self.batch = B = int(batch)
+        self.bstride = bs = stride_for(B)
+        self.nb = nb = n * bs
+        self.n_waves = int(self.lib.fis_n_waves(n, bs))
+        self.ev_cap = L.MAX_WINDOW * (1 if bs >= 64 else 64 // bs)
+        dev = net.device
+        self.device = dev
+        z = dict(device=dev)
+        self.v = torch.full((nb,), float(self.c["v0"]), dtype=torch.float32, **z)
+        self.g = torch.zeros(nb, dtype=torch.float32, **z)
+        self.st = torch.zeros(nb, dtype=torch.uint8, **z)
+        self.fl = torch.zeros(nb, dtype=torch.uint8, **z)
+        self.vhold = None
+        self.ring = torch.zeros(L.RING_LEN * nb, dtype=torch.int32, **z)
+        self.sbits = torch.zeros(nb, dtype=torch.int32, **z)
+        self.events = torch.zeros(self.n_waves * self.ev_cap * 2,
+                                  dtype=torch.int64, **z)
+        self.ev_count = torch.zeros(self.n_waves, dtype=torch.int32, **z)
+        self.clock = torch.zeros(1, dtype=torch.int64, **z)
+        self.done = torch.zeros(1, dtype=torch.int32, **z)
+        sd = np.zeros(bs, np.uint64)
+        sd[:B] = 0 if seeds is None else np.asarray(seeds, np.uint64)
+        self.seeds_host = sd
+        self.seeds = torch.from_numpy(sd.view(np.int64)).to(dev)
+        ti = np.zeros(bs, np.uint32)
+        ti[:B] = np.arange(B) if trial_ids is None else np.asarray(trial_ids,
+                                                                  np.uint32)
+        self.trial_ids_host = ti
+        self.trial_ids = torch.from_numpy(ti.view(np.int32)).to(dev)
+        self.spikes_out = (torch.zeros(B * n, dtype=torch.float32, **z)
+                           if spikes_out else None)
+        self.counts = torch.zeros(nb, dtype=torch.int32, **z) if counts else None
+        self.pop_counts = (torch.zeros(n, dtype=torch.int32, **z)
+                           if pop_counts else None)
+        self.raster_len = int(raster_len)
+        self.raster = (torch.zeros(self.raster_len * self.n_waves,
+                                   dtype=torch.int64, **z)
+                       if raster_len else None)
+        self.t = 0  # host mirror of the device clock
+
+        # refractory period per neuron-trial and padding trials
+        fl = self.fl.view(n, bs)
+        if bs > B:
+            fl[:, B:] = L.FIRE_OFF
+        self.set_norefrac(no_refractory)
+        self._sync_struct()
+
+    # ------------------------------------------------------------ struct
+    def _sync_struct(self):
+        c = self.c
+        s = L.FisSim()
+        s.n, s.batch, s.bstride, s.ev_cap = self.n, self.batch, self.bstride, \
+            self.ev_cap
+        s.offsets = self.net.offsets.data_ptr()
+        s.cols = self.net.cols.data_ptr()
+        s.vals = self.net.vals.data_ptr()
+        for name in ("v", "g", "st", "fl", "ring", "sbits", "events",
+                     "ev_count", "clock", "done", "seeds", "trial_ids"):
+            setattr(s, name, getattr(self, name).data_ptr())
+        s.vhold = self.vhold.data_ptr() if self.vhold is not None else None
+        for name in ("w_scale", "a_syn", "a_mem", "v_rest", "v_reset", "v_th",
+                     "gain", "prob_scale"):
+            setattr(s, name, float(c[name]))
+        s.refrac_steps = int(c["refrac_steps"])
+        for name in ("spikes_out", "counts", "pop_counts", "raster"):
+            t = getattr(self, name)
+            setattr(s, name, t.data_ptr() if t is not None else None)
+        s.raster_len = self.raster_len
+        self.struct = s
+        self._sp = ctypes.byref(s)
+
+    # ------------------------------------------------------------ flags
+    def _trials(self, trials):
+        if trials is None:
+            return np.arange(self.batch)
+        t = np.asarray(trials, dtype=np.int64).reshape(-1)
+        if t.size and (t.min() < 0 or t.max() >= self.batch):
+            raise IndexError("trial index outside [0, %d)" % self.batch)
+        return t
+
+    def _neurons(self, idx):
+        a = np.asarray(idx if idx is not None else [], dtype=np.int64).reshape(-1)
+        if a.size and (a.min() < 0 or a.max() >= self.n):
+            raise IndexError("neuron index outside [0, %d)" % self.n)
+        return a
+
+    def set_flag(self, bit, neurons, trials=None):
+        """Clear ``bit`` on the given trials, then set it on ``neurons``."""
+        tr = self._trials(trials)
+        ne = self._neurons(neurons)
+        fl = self.fl.view(self.n, self.bstride)
+        tt = torch.as_tensor(tr, device=self.device)
+        cur = fl[:, tt]
+        fl[:, tt] = cur & (0xFF ^ bit)
+        if ne.size:
+            nn = torch.as_tensor(ne, device=self.device)
+            sub = fl[nn][:, tt]
+            fl[nn.unsqueeze(1), tt.unsqueeze(0)] = sub | bit
+
+    def set_flag_pairs(self, bit, neurons, trials):
+        """Set ``bit`` on explicit (neuron, trial) pairs (other entries kept)."""
+        ne = self._neurons(neurons)
+        tr = np.asarray(trials, dtype=np.int64).reshape(-1)
+        if ne.size != tr.size:
+            raise ValueError("neurons and trials differ in length")
+        if tr.size and (tr.min() < 0 or tr.max() >= self.batch):
+            raise IndexError("trial index outside [0, %d)" % self.batch)
+        if ne.size:
+            idx = torch.as_tensor(ne * self.bstride + tr, device=self.device)
+            self.fl[idx] = self.fl[idx] | bit
+
+    def set_norefrac(self, neurons, trials=None):
+        """Refractory period 0 for ``neurons`` (sim.py:371-381)."""
+        self.set_flag(L.NOREFRAC, neurons, trials)
+        # the initial counter value is refrac_steps (0 for exempt neurons),
+        # as Engine.__init__ sets refrac = refrac_steps (sim.py:388)
+        R = int(self.c["refrac_steps"])
+        fl = self.fl.view(self.n, self.bstride)
+        st = self.st.view(self.n, self.bstride)
+        fresh = (st == 0) | (st == R)
+        if self.t == 0:
+            free = (fl & L.NOREFRAC) != 0
+            st[fresh & free] = 0
+            st[fresh & ~free] = R
+
+    def set_hold(self, neurons, v_clamp, trials=None):
+        self.set_flag(L.HOLD, neurons, trials)
+        if self.vhold is None:
+            self.vhold = torch.zeros(self.nb, dtype=torch.float32,
+                                     device=self.device)
+            self._sync_struct()
+        tr = self._trials(trials)
+        ne = self._neurons(neurons)
+        vh = self.vhold.view(self.n, self.bstride)
+        tt = torch.as_tensor(tr, device=self.device)
+        vh[:, tt] = 0.0
+        if ne.size:
+            nn = torch.as_tensor(ne, device=self.device)
+            vh[nn.unsqueeze(1), tt.unsqueeze(0)] = float(np.float32(v_clamp))
+
+    # ------------------------------------------------------------ launches
+    def drive_rates(self, rates, W=1):
+        """Kicks for W steps from dense rates [batch, n] (Hz) on the device."""
+        if rates.dtype != torch.float32 or not rates.is_contiguous() or \
+                tuple(rates.shape) != (self.batch, self.n) or \
+                rates.device != self.device:
+            raise ValueError("rates must be contiguous float32 [%d, %d] on %s"
+                             % (self.batch, self.n, self.device))
+        L.check(self.lib.fis_drive_rates(self._sp, _ptr(rates), W, _stream()),
+                "fis_drive_rates")
+
+    def drive_table(self, table, W):
+        L.check(self.lib.fis_drive_table(
+            self._sp, _ptr(table.entries), table.n_entries,
+            _ptr(table.rate_hz), table.n_groups, table.n_seg, table.seg_steps,
+            W, _stream()), "fis_drive_table")
+
+    def drive_tape(self, tape, W):
+        L.check(self.lib.fis_drive_tape(self._sp, _ptr(tape.ptr),
+                                        _ptr(tape.idx), tape.n_steps, W,
+                                        _stream()), "fis_drive_tape")
+
+    def update(self, W):
+        L.check(self.lib.fis_update(self._sp, W, _stream()), "fis_update")
+        self.t += W
+
+    def scatter(self):
+        L.check(self.lib.fis_scatter(self._sp, _stream()), "fis_scatter")
+
+    def window(self, W, drive=None):
+        """One window: drive (a DriveTable, Tape or rates tensor), update,
+        scatter."""
+        if not 1 <= W <= L.MAX_WINDOW:
+            raise ValueError("window of %d steps (1..%d)" % (W, L.MAX_WINDOW))
+        if drive is not None:
+            if isinstance(drive, torch.Tensor):
+                self.drive_rates(drive, W)
+            elif isinstance(drive, DriveTable):
+                self.drive_table(drive, W)
+            elif isinstance(drive, Tape):
+                self.drive_tape(drive, W)
+            else:
+                raise TypeError("unknown drive %r" % type(drive))
+        self.update(W)
+        self.scatter()
+
+    def seed_ring(self, slot):
+        L.check(self.lib.fis_seed_ring(self._sp, slot, _stream()),
+                "fis_seed_ring")
+
+    # ------------------------------------------------------------ views
+    def view(self, t):
+        """[batch, n] view of a trial-minor [n * bstride] tensor."""
+        return t.view(self.n, self.bstride)[:, :self.batch].t()
+
+    def refrac_counter(self):
+        return self.view(self.st) & L.ST_RMAX
+
+    def spike_state(self):
+        return (self.view(self.st) & L.ST_SPIKE) != 0
+
+    def ring_counts(self, step_input):
+        """Integer synaptic input that step ``step_input`` reads, i.e.
+        the sum computed from the spikes of step ``step_input - 20``;
+        [batch, n]."""
+        slot = (step_input + 1) % L.RING_LEN
+        return self.view(self.ring[slot * self.nb:(slot + 1) * self.nb])
+
+    def read_raster(self, t_from, t_to):
+        """Spikes of steps [t_from, t_to) from the raster ring, bool
+        [steps, batch, n].  Rows older than raster_len steps are gone."""
+        if self.raster is None:
+            raise RuntimeError("built without a raster")
+        if t_to - t_from > self.raster_len or t_from < self.t - self.raster_len:
+            raise ValueError("raster rows [%d, %d) no longer held"
+                             % (t_from, t_to))
+        rows = [t % self.raster_len for t in range(t_from, t_to)]
+        r = self.raster.view(self.raster_len, self.n_waves)[rows].cpu().numpy()
+        bits = np.unpackbits(r.view(np.uint8), axis=1, bitorder="little")
+        bits = bits[:, :self.nb].reshape(len(rows), self.n, self.bstride)
+        return bits[:, :, :self.batch].transpose(0, 2, 1).astype(bool)
+
+    def rec_slot(self, u):
+        """The reference's rec(u) (sum of spikes of step u-1), int [B, n]."""
+        slot = u % L.RING_LEN
+        return self.view(self.ring[slot * self.nb:(slot + 1) * self.nb])
+
+
+class DriveTable:
+    """Poisson-driven neuron-trials with a piecewise-constant rate schedule.
+
+    ``idx`` are neuron-trial indices (neuron * bstride + trial), ``group``
+    picks a column of ``rate_hz[n_seg, n_groups]``; segment = step //
+    seg_steps (the last segment holds afterwards).
+    """
+
+    def __init__(self, sim: DeviceSim, neurons, trials, groups, rate_hz,
+                 seg_steps=1 << 30):
+        ne = np.asarray(neurons, dtype=np.int64).reshape(-1)
+        tr = np.asarray(trials, dtype=np.int64).reshape(-1)
+        gr = np.asarray(groups, dtype=np.int64).reshape(-1)
+        rt = np.asarray(rate_hz, dtype=np.float32)
+        if rt.ndim == 1:
+            rt = rt[None, :]
+        if not (ne.size == tr.size == gr.size):
+            raise ValueError("neurons, trials and groups differ in length")
+        if ne.size:
+            if ne.min() < 0 or ne.max() >= sim.n:
+                raise IndexError("driven neuron outside [0, %d)" % sim.n)
+            if tr.min() < 0 or tr.max() >= sim.batch:
+                raise IndexError("driven trial outside [0, %d)" % sim.batch)
+            if gr.min() < 0 or gr.max() >= rt.shape[1]:
+                raise IndexError("rate group outside [0, %d)" % rt.shape[1])
+        idx = ne * sim.bstride + tr
+        if np.unique(idx).size != idx.size:
+            raise ValueError("a neuron-trial appears twice in the drive table")
+        check_rates_hz(rt, "drive table")
+        ent = np.empty((idx.size, 2), np.uint32)
+        ent[:, 0] = idx
+        ent[:, 1] = gr
+        self.n_entries = int(idx.size)
+        self.n_seg, self.n_groups = int(rt.shape[0]), int(rt.shape[1])
+        self.seg_steps = int(min(seg_steps, 1 << 30))
+        dev = sim.device
+        self.entries = torch.from_numpy(ent.view(np.int32).reshape(-1)).to(dev) \
+            if idx.size else torch.zeros(2, dtype=torch.int32, device=dev)
+        self.rate_hz = torch.from_numpy(np.ascontiguousarray(rt).reshape(-1)).to(dev)
+
+    def set_rates(self, rate_hz):
+        rt = np.asarray(rate_hz, dtype=np.float32).reshape(self.n_seg,
+                                                           self.n_groups)
+        check_rates_hz(rt, "drive table")
+        self.rate_hz.copy_(torch.from_numpy(rt.reshape(-1)))
+
+
+class Tape:
+    """Recorded Poisson outcomes: kicked neuron-trial indices per step."""
+
+    def __init__(self, sim: DeviceSim, steps, neurons, trials=None):
+        st = np.asarray(steps, dtype=np.int64).reshape(-1)
+        ne = np.asarray(neurons, dtype=np.int64).reshape(-1)
+        tr = np.zeros_like(ne) if trials is None else \
+            np.asarray(trials, dtype=np.int64).reshape(-1)
+        if ne.size and (ne.min() < 0 or ne.max() >= sim.n or tr.min() < 0
+                        or tr.max() >= sim.batch or st.min() < 0):
+            raise IndexError("tape event outside the simulation")
+        self.n_steps = int(st.max()) + 1 if st.size else 0
+        order = np.argsort(st, kind="stable")
+        st, ne, tr = st[order], ne[order], tr[order]
+        ptr = np.zeros(self.n_steps + 1, np.int64)
+        np.cumsum(np.bincount(st, minlength=self.n_steps), out=ptr[1:])
+        idx = (ne * sim.bstride + tr).astype(np.int32)
+        dev = sim.device
+        self.ptr = torch.from_numpy(ptr).to(dev)
+        self.idx = torch.from_numpy(idx if idx.size else
+                                    np.zeros(1, np.int32)).to(dev)
+
+
+MAX_RATE_HZ = 10000.0  # 1000 / DT (lab/flybrain/sim.py:131)
+
+
+class RateTooHigh(ValueError):
+    """A rate outside [0, 1000/DT] Hz (lab/flybrain/sim.py:134-135)."""
+
+
+def check_rates_hz(values, where):
+    """Reject rates outside [0, 10,000] Hz before they reach a kernel.
+
+    Same bounds and same loud failure as ``check_rates_hz``
+    (lab/flybrain/sim.py:138-161): at DT = 0.1 ms a probability above one
+    has no meaning, and the check belongs to whoever writes rates.
+    """
+    a = np.asarray(values, dtype=np.float64)
+    if a.size == 0:
+        return values
+    worst = float(np.nanmax(a))
+    lowest = float(np.nanmin(a))
+    if not (lowest >= 0.0) or worst > MAX_RATE_HZ:
+        raise RateTooHigh(
+            "%s: drive rate %.4g Hz (lowest %.4g) outside [0, %.0f]; the "
+            "Poisson input is Bernoulli(rate * DT/1000) and needs a "
+            "probability in [0, 1]" % (where, worst, lowest, MAX_RATE_HZ))
+    return values

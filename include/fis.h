@@ -1,0 +1,181 @@
+/*
+ * fis.h -- C ABI of the MI355X LIF engine library (libfis.so).
+ *
+ * Every entry point takes raw device pointers and sizes, takes the HIP
+ * stream as `void*` (0 = the null stream), only enqueues work (no host
+ * synchronisation, so every call can be captured into a hipGraph) and
+ * returns a hipError_t as int (0 = success).  The library never allocates:
+ * the caller owns every buffer.
+ *
+ * Two groups of entry points:
+ *
+ *  1. fis_ev_spmv_i32 / fis_ev_spmv_i64 / fis_ev_spmv_abi -- drop-in twins
+ *     of the reference's event SpMV C ABI, lab/flybrain/ev_spmv.cu:114-137
+ *     (ctypes signatures lab/flybrain/evspmv.py:896-903):
+ *         out[b, post] += sum_pre W[post, pre] * sp[b, pre] * mo[b, pre]
+ *     over a CSR of W^T (row = presynaptic neuron), float atomics.
+ *
+ *  2. The fused engine (replaces Engine._core + Engine._poisson,
+ *     lab/flybrain/sim.py:848-953, and the csrmv / event roads it calls):
+ *     per window of W <= FIS_MAX_WINDOW steps
+ *         fis_drive_*   Poisson kicks for the window into `sbits`
+ *         fis_update    W steps of the membrane / conductance / refractory
+ *                       update per neuron-trial, spike compaction into
+ *                       per-wave event segments
+ *         fis_scatter   event-driven scatter of the window's spikes into
+ *                       the int32 delay ring (exact integer sums)
+ */
+#ifndef FIS_H
+#define FIS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FIS_ABI_VERSION 1
+#define FIS_MAX_WINDOW 20   /* steps per window: delay horizon 19 + 1     */
+#define FIS_RING_LEN 20     /* int32 ring slots: rec(t-19) .. rec(t)       */
+
+/* flag bits of `fl`, one byte per neuron-trial (sim.py:393-410, :866-900) */
+#define FIS_FIRE_OFF 0x01u  /* m_fire == 0: spike output suppressed        */
+#define FIS_OUT_OFF 0x02u   /* m_out  == 0: spike not transmitted          */
+#define FIS_IN_OFF 0x04u    /* m_in   == 0: delayed input not received     */
+#define FIS_HOLD 0x08u      /* m_hold == 0: v replaced by vhold            */
+#define FIS_NOREFRAC 0x10u  /* refractory period 0 (driven neuron)         */
+
+/* `st`, one byte per neuron-trial: bit 7 = spike of the last step, bits
+ * 0..6 = steps since the last spike, saturating at 127 (the refractory
+ * gate only compares it with refrac_steps <= 126). */
+#define FIS_ST_SPIKE 0x80u
+#define FIS_ST_RMAX 127
+
+/* One spike event: neuron `pre` spiked at the step whose input lands in
+ * ring slot `slot`, in the trials tb + j for every set bit j of `mask`. */
+typedef struct {
+  int32_t pre;
+  uint16_t slot;
+  uint16_t tb;
+  uint64_t mask;
+} fis_event;
+
+/*
+ * Everything the engine kernels touch.  Neuron-trial index
+ * idx = neuron * bstride + trial; bstride is 1, 2, 4, ..., 64 or a multiple
+ * of 64 (trials >= batch are padding with FIS_FIRE_OFF set).
+ */
+typedef struct {
+  int32_t n;         /* neurons                                        */
+  int32_t batch;     /* trials in use                                  */
+  int32_t bstride;   /* trial stride (padded batch)                    */
+  int32_t ev_cap;    /* event capacity per update wave                 */
+  /* connectome by presynaptic neuron (W^T CSR) */
+  const int64_t* offsets; /* [n+1]                                     */
+  const int32_t* cols;    /* [E] postsynaptic neuron                   */
+  const int16_t* vals;    /* [E] signed synapse count                  */
+  /* state, [n * bstride] each */
+  float* v;
+  float* g;
+  uint8_t* st;
+  const uint8_t* fl;
+  const float* vhold;     /* nullable; read where FIS_HOLD is set      */
+  int32_t* ring;          /* [FIS_RING_LEN][n * bstride]               */
+  uint32_t* sbits;        /* [n * bstride] kick bits of the window     */
+  fis_event* events;      /* [n_waves * ev_cap]                        */
+  int32_t* ev_count;      /* [n_waves]                                 */
+  int64_t* clock;         /* [1] first step of the next window         */
+  uint32_t* done;         /* [1] completion counter, starts at 0       */
+  /* per-trial Philox key and stream id, [bstride] */
+  const uint64_t* seeds;
+  const uint32_t* trial_ids;
+  /* float32 constants, already rounded the way the reference's tensor
+   * ops round Python scalars (sim.py:361-383) */
+  float w_scale;     /* wScale                       0.275           */
+  float a_syn;       /* 1 - DT/tauSyn                0.98            */
+  float a_mem;       /* DT/tauMem                    0.005           */
+  float v_rest;      /* vRest                        -52             */
+  float v_reset;     /* vReset                       -52             */
+  float v_th;        /* vThreshold                   -45             */
+  float gain;        /* scalePoisson                 250             */
+  float prob_scale;  /* DT/1000                      1e-4            */
+  int32_t refrac_steps; /* round(tRefrac/DT)         22              */
+  /* optional outputs (nullable) */
+  float* spikes_out;    /* [batch][n] float 0/1 of the window's last step */
+  uint32_t* counts;     /* [n * bstride] spike count per neuron-trial  */
+  uint32_t* pop_counts; /* [n] spike count per neuron, all trials      */
+  uint64_t* raster;     /* [raster_len][n_waves] ballot of every spike,
+                           row (t mod raster_len)                      */
+  int32_t raster_len;
+} fis_sim;
+
+/* Drive table entry: neuron-trial `idx` is driven at rate
+ * rate_hz[segment][group] with segment = min(t / seg_steps, n_seg - 1). */
+typedef struct {
+  uint32_t idx;
+  uint32_t group;
+} fis_drive_entry;
+
+int fis_abi_version(void);
+
+/* ---- reference-ABI twins (lab/flybrain/ev_spmv.cu:114-137) ---------- */
+
+/* ev_spmv_i32 (ev_spmv.cu:114-121): int32 W^T CSR.  `threads` is accepted
+ * for signature compatibility; the launch shape is the library's own. */
+int fis_ev_spmv_i32(const int32_t* crow, const int32_t* col, const float* val,
+                    const float* sp, const float* mo, float* out, int n,
+                    int batch, int threads, void* stream);
+/* ev_spmv_i64 (ev_spmv.cu:123-130): int64 W^T CSR. */
+int fis_ev_spmv_i64(const int64_t* crow, const int64_t* col, const float* val,
+                    const float* sp, const float* mo, float* out, int n,
+                    int batch, int threads, void* stream);
+/* ev_spmv_abi (ev_spmv.cu:136-137) returns 3 in the reference; this
+ * library's twin returns the same number: the argument list above is that
+ * ABI, unchanged. */
+int fis_ev_spmv_abi(void);
+
+/* ---- the fused engine ----------------------------------------------- */
+
+/* Poisson kicks for a window of W steps from dense rates [batch][n] in Hz,
+ * constant over the window (Engine._poisson, sim.py:848-861, fed by the
+ * `rates` argument of Engine.step, sim.py:745-771). */
+int fis_drive_rates(const fis_sim* s, const float* rates, int W, void* stream);
+
+/* Poisson kicks from a drive table: entries [n_entries], rate table
+ * rate_hz[n_seg][n_groups] (piecewise-constant schedule, seg_steps steps
+ * per segment). */
+int fis_drive_table(const fis_sim* s, const fis_drive_entry* entries,
+                    int n_entries, const float* rate_hz, int n_groups,
+                    int n_seg, int seg_steps, int W, void* stream);
+
+/* Kicks from a recorded tape: for step t the neuron-trial indices
+ * tape_idx[tape_ptr[t] .. tape_ptr[t+1]), t in [0, n_tape_steps). */
+int fis_drive_tape(const fis_sim* s, const int64_t* tape_ptr,
+                   const int32_t* tape_idx, int64_t n_tape_steps, int W,
+                   void* stream);
+
+/* W steps of the neuron update (Engine._core minus the SpMV,
+ * sim.py:920-953), spike compaction into events, clock += W. */
+int fis_update(const fis_sim* s, int W, void* stream);
+
+/* Scatter of the last window's events into the int32 ring
+ * (the SpMV of sim.py:906-918 / ev_spmv.cu:60-79, integer, exact). */
+int fis_scatter(const fis_sim* s, void* stream);
+
+/* Scatter the current spike state (st bit 7, not FIS_OUT_OFF) into ring
+ * slot `slot` -- used after a restore to rebuild the input in flight. */
+int fis_seed_ring(const fis_sim* s, int slot, void* stream);
+
+/* sizeof / offsetof of the structs above (0: sizeof fis_sim, 1: offsets,
+ * 2: w_scale, 3: refrac_steps, 4: raster_len, 5: sizeof fis_event,
+ * 6: sizeof fis_drive_entry), so a binding can check its mirror. */
+int64_t fis_struct_layout(int what);
+
+/* Number of update waves (event segments) for a given n and bstride. */
+int64_t fis_n_waves(int32_t n, int32_t bstride);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FIS_H */

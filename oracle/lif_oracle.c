@@ -115,6 +115,9 @@ typedef struct {
 
 /*
  * One step for all trials, in the reference's operation order.
+ * refrac_steps is [N] like Engine.refrac_steps, or [B][N] when
+ * rs_per_trial is set (trials with their own driven sets, each trial then
+ * being an independent reference run with its own no_refractory).
  * Masks may be NULL (all ones / v_hold zero).  rec_out (may be NULL)
  * receives the synaptic input written into the ring this step, and
  * icount_out (may be NULL) the same as the exact integer sum.
@@ -125,7 +128,8 @@ typedef struct {
 int oracle_step(int n, int batch, const int64_t* offsets, const int32_t* cols,
                 const int16_t* vals, const oracle_params* prm, float* v,
                 float* g, float* spikes, float* refrac, float* ring, int L,
-                int ptr, const float* refrac_steps, const float* m_fire,
+                int ptr, const float* refrac_steps, int rs_per_trial,
+                const float* m_fire,
                 const float* m_out, const float* m_in, const float* m_hold,
                 const float* v_hold, const float* stim, float* rec_out,
                 int32_t* icount_out, int64_t* acc) {
@@ -158,7 +162,8 @@ int oracle_step(int n, int batch, const int64_t* offsets, const int32_t* cols,
       float one_minus_s = 1.0f - spikes[k];
       r = r * one_minus_s;
       refrac[k] = r;
-      float gate = (r >= refrac_steps[i]) ? 1.0f : 0.0f;
+      float gate = (r >= refrac_steps[rs_per_trial ? k : (size_t)i]) ? 1.0f
+                                                                     : 0.0f;
 
       /* delayed input, read-then-write of ring slot ptr */
       float* slot = ring + ((size_t)b * L + ptr) * n;

@@ -65,8 +65,8 @@ def lib():
                                        ctypes.c_float, ctypes.c_float, P]
         L.oracle_stim_tape.restype = None
         L.oracle_step.argtypes = ([ctypes.c_int, ctypes.c_int, P, P, P, P]
-                                  + [P] * 5 + [ctypes.c_int, ctypes.c_int]
-                                  + [P] * 10)
+                                  + [P] * 5 + [ctypes.c_int, ctypes.c_int, P,
+                                               ctypes.c_int] + [P] * 9)
         L.oracle_step.restype = ctypes.c_int
         L.oracle_abi.restype = ctypes.c_int
         _lib = L
@@ -120,7 +120,7 @@ class OracleEngine:
     """State and step of the reference engine, trial-major [B, N]."""
 
     def __init__(self, conn, batch=1, params=None, no_refractory=None,
-                 seeds=None, trial_ids=None):
+                 seeds=None, trial_ids=None, no_refractory_trials=None):
         self.conn = conn
         self.n = n = conn.n
         self.batch = B = int(batch)
@@ -129,10 +129,19 @@ class OracleEngine:
         self.refrac_steps = np.full(n, self.c["refrac_steps"], np.float32)
         if no_refractory is not None and len(no_refractory):
             self.refrac_steps[np.asarray(no_refractory, dtype=np.int64)] = 0.0
+        if no_refractory_trials is not None:
+            # per-trial exemption: B independent single-trial references
+            rs = np.tile(self.refrac_steps, (B, 1)).astype(np.float32)
+            for b, idx in enumerate(no_refractory_trials):
+                if len(idx):
+                    rs[b, np.asarray(idx, dtype=np.int64)] = 0.0
+            self.refrac_steps = rs
         self.v = np.full((B, n), self.c["v0"], np.float32)
         self.conductance = np.zeros((B, n), np.float32)
         self.spikes = np.zeros((B, n), np.float32)
-        self.refrac = np.tile(self.refrac_steps, (B, 1)).astype(np.float32)
+        self.refrac = (np.tile(self.refrac_steps, (B, 1)).astype(np.float32)
+                       if self.refrac_steps.ndim == 1
+                       else self.refrac_steps.copy())
         self.delay_buffer = np.zeros((B, self.buf_len, n), np.float32)
         self.ptr = 0
         self.m_fire = self.m_out = self.m_in = self.m_hold = None
@@ -203,7 +212,8 @@ class OracleEngine:
             self.n, self.batch, _p(cn.offsets), _p(cn.cols), _p(cn.vals),
             ctypes.byref(self._prm), _p(self.v), _p(self.conductance),
             _p(self.spikes), _p(self.refrac), _p(self.delay_buffer),
-            self.buf_len, self.ptr, _p(self.refrac_steps), opt(self.m_fire),
+            self.buf_len, self.ptr, _p(self.refrac_steps),
+            int(self.refrac_steps.ndim == 2), opt(self.m_fire),
             opt(self.m_out), opt(self.m_in), opt(self.m_hold),
             opt(self.v_hold), _p(stim), _p(self.rec), _p(self.icount),
             _p(self._acc))

@@ -245,6 +245,7 @@ def run_sched(S):
     np.savez_compressed(
         HERE / "run_sched.npz",
         conn_fp=np.array(conn.fingerprint()), n=np.int64(n), batch=np.int64(B),
+        conn_offsets=conn.offsets, conn_cols=conn.cols, conn_vals=conn.vals,
         stim_a=a, stim_b=b, silenced=sil, t_run_s=np.float64(0.06),
         bin_ms=np.float64(2.5), tape_bits=np.array(tape),
         rates_a=np.array(rates_a, np.float32),

@@ -1,0 +1,215 @@
+"""Parity of the HIP path with the oracle and the reference's golden vectors.
+
+Bar (integer/byte work bit-exact; float32 state within 1e-4 mV, measured
+here as bit-exact because every float op keeps the reference's rounding):
+  * per-step synaptic input (integer synapse counts) identical;
+  * spike raster identical;
+  * v and g bit-identical, refractory counter = min(reference, 127).
+"""
+
+from __future__ import annotations
+
+import hashlib
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from flybrain_amd import connectome as C
+from flybrain_amd.device import DeviceNet, DeviceSim, DriveTable, Tape
+from gpu_helpers import (apply_masks_device, compare_state, mask_schedule,
+                         state_of, tiny_conn)
+
+pytestmark = pytest.mark.gpu
+
+
+def fp(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()[:16]
+
+
+def tape_of_bits(bits):
+    """(steps, neurons, trials) of a [steps, B, n] 0/1 array."""
+    t, b, i = np.nonzero(bits)
+    return t, i, b
+
+
+@pytest.fixture(scope="module")
+def tiny(golden):
+    z = golden("tiny_masks.npz")
+    conn = tiny_conn(z)
+    n = conn.n
+    bits = np.unpackbits(z["tape_bits"], axis=-1)[..., :n]
+    return z, conn, bits
+
+
+def test_tiny_masks_per_step(tiny):
+    """W=1 steps, masks switched as in the fixture: every step equals the
+    oracle and the reference's own fingerprints of v, g and spikes."""
+    z, conn, bits = tiny
+    B, steps = int(z["batch"]), int(z["steps"])
+    net = DeviceNet(conn)
+    sim = DeviceSim(net, B, no_refractory=z["no_refractory"], raster_len=32)
+    t_, i_, b_ = tape_of_bits(bits)
+    tape = Tape(sim, t_, i_, b_)
+    ora = O.OracleEngine(conn, B, no_refractory=z["no_refractory"])
+    sched = mask_schedule(z)
+    for t in range(steps):
+        if t in sched:
+            apply_masks_device(sim, sched[t])
+            ora.set_masks(**sched[t])
+        sim.window(1, tape)
+        ora.step_bits(bits[t])
+        v, g, sp, r = state_of(sim)
+        assert fp(v) == z["fps"][t][0], "step %d: v fingerprint" % t
+        assert fp(g) == z["fps"][t][1], "step %d: g fingerprint" % t
+        assert fp(sp) == z["fps"][t][2], "step %d: spikes fingerprint" % t
+        compare_state(sim, ora, "step %d" % t)
+        got = sim.rec_slot(t).cpu().numpy()
+        np.testing.assert_array_equal(got, ora.icount,
+                                      err_msg="step %d: synaptic input" % t)
+
+
+def test_tiny_masks_windows(tiny):
+    """Windows of up to 20 steps between mask changes: raster and state at
+    every window end equal the oracle; synaptic input equals the golden."""
+    z, conn, bits = tiny
+    B, steps = int(z["batch"]), int(z["steps"])
+    net = DeviceNet(conn)
+    sim = DeviceSim(net, B, no_refractory=z["no_refractory"], raster_len=64)
+    t_, i_, b_ = tape_of_bits(bits)
+    tape = Tape(sim, t_, i_, b_)
+    ora = O.OracleEngine(conn, B, no_refractory=z["no_refractory"])
+    sched = mask_schedule(z)
+    cuts = sorted(set(range(0, steps, 20)) | set(sched) | {steps})
+    rec_golden = z["rec"]
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        while a < b:
+            W = min(20, b - a)
+            if a in sched:
+                apply_masks_device(sim, sched[a])
+                ora.set_masks(**sched[a])
+            sim.window(W, tape)
+            raster = sim.read_raster(a, a + W)
+            for k in range(W):
+                ora.step_bits(bits[a + k])
+                assert np.array_equal(raster[k], ora.spikes.astype(bool)), \
+                    "raster differs at step %d" % (a + k)
+            e = a + W - 1
+            for u in range(max(a, e - 18), e + 1):
+                # the ring still holds rec(u) for the last 19 steps
+                want = np.rint(rec_golden[u] / np.float32(0.275))
+                np.testing.assert_array_equal(
+                    sim.rec_slot(u).cpu().numpy(), want.astype(np.int32),
+                    err_msg="synaptic input of step %d" % u)
+            compare_state(sim, ora, "window end %d" % e)
+            a += W
+
+
+def test_small10k_config1(golden):
+    """10,000 neurons / 1.09M edges, 20 neurons at 150 Hz from the recorded
+    tape, 1,000 steps in 20-step windows: raster and final state equal the
+    reference's golden vectors."""
+    z = golden("small10k.npz")
+    conn = C.small()
+    assert conn.fingerprint() == str(z["conn_fp"])
+    steps = int(z["steps"])
+    net = DeviceNet(conn)
+    sim = DeviceSim(net, 1, no_refractory=z["driven"], raster_len=1000)
+    tape = Tape(sim, z["tape_t"], z["tape_i"])
+    for a in range(0, steps, 20):
+        sim.window(20, tape)
+    raster = sim.read_raster(0, steps)
+    t, _, i = np.nonzero(raster)
+    assert t.size == z["spike_t"].size
+    assert np.array_equal(t, z["spike_t"]) and np.array_equal(i, z["spike_i"])
+    v, g, sp, r = state_of(sim)
+    assert fp(v) == z["fps"][-1][0]
+    assert fp(g) == z["fps"][-1][1]
+    np.testing.assert_array_equal(r[0], np.minimum(z["refrac_final"][0], 127))
+
+
+@pytest.mark.parametrize("batch", [3, 64, 130])
+def test_philox_drive_batched(batch):
+    """Dense-rate Philox drive (Engine.step road) and per-trial drive tables
+    against the oracle running the same Philox stream, trial by trial."""
+    conn = C.synthetic(512, None, mean_out=40, count_exp=1.2, n_groups=4,
+                       seed=21)
+    n = conn.n
+    rng = np.random.default_rng(batch)
+    seeds = rng.integers(0, 2**63, size=batch, dtype=np.uint64)
+    drv = [rng.choice(n, rng.integers(5, 40), replace=False) for _ in range(batch)]
+    rate = [float(rng.uniform(50, 900)) for _ in range(batch)]
+    net = DeviceNet(conn)
+    sim = DeviceSim(net, batch, seeds=seeds, raster_len=40)
+    ne = np.concatenate(drv)
+    tr = np.concatenate([np.full(len(d), b) for b, d in enumerate(drv)])
+    sim.set_flag_pairs(0x10, ne, tr)  # refractory exemption per trial
+    sim.st.view(n, sim.bstride)[torch.as_tensor(ne), torch.as_tensor(tr)] = 0
+    table = DriveTable(sim, ne, tr, tr, np.array(rate, np.float32))
+    ora = O.OracleEngine(conn, batch, seeds=seeds, no_refractory_trials=drv)
+    rates = np.zeros((batch, n), np.float32)
+    for b, d in enumerate(drv):
+        rates[b, d] = rate[b]
+    steps = 400
+    for a in range(0, steps, 20):
+        sim.window(20, table)
+        raster = sim.read_raster(a, a + 20)
+        for k in range(20):
+            ora.step_rates(rates)
+            assert np.array_equal(raster[k], ora.spikes.astype(bool)), \
+                "raster differs at step %d" % (a + k)
+    compare_state(sim, ora, "end")
+
+
+def test_rates_drive_matches_table():
+    """fis_drive_rates (dense [B, n] rates, Engine.step road) produces the
+    same kicks as a drive table of the same rates."""
+    conn = C.synthetic(300, None, mean_out=30, n_groups=0, seed=3)
+    n, B = conn.n, 2
+    net = DeviceNet(conn)
+    a = DeviceSim(net, B, seeds=[11, 12], raster_len=8)
+    b = DeviceSim(net, B, seeds=[11, 12], raster_len=8)
+    rates = torch.zeros(B, n, device="cuda")
+    idx = np.arange(0, n, 7)
+    rates[0, idx] = 2000.0
+    rates[1, idx[::2]] = 7000.0
+    ne = np.concatenate([idx, idx[::2]])
+    tr = np.concatenate([np.zeros(idx.size), np.ones(idx[::2].size)])
+    table = DriveTable(a, ne, tr.astype(np.int64), np.arange(ne.size),
+                       rates[tr.astype(np.int64), ne].cpu().numpy())
+    for t in range(60):
+        a.window(1, rates)
+        b.window(1, table)
+        assert np.array_equal(a.read_raster(t, t + 1), b.read_raster(t, t + 1))
+    torch.testing.assert_close(a.v, b.v, rtol=0, atol=0)
+
+
+def test_ev_spmv_twin_matches_dense():
+    """fis_ev_spmv_i32/i64 (the reference's ev_spmv ABI) against a dense
+    float64 product on the CPU."""
+    import ctypes
+
+    from flybrain_amd import _lib as L
+
+    conn = C.synthetic(700, None, mean_out=50, n_groups=0, seed=5)
+    n, B = conn.n, 3
+    rng = np.random.default_rng(0)
+    sp = (rng.random((B, n)) < 0.2).astype(np.float32)
+    mo = (rng.random((B, n)) < 0.9).astype(np.float32)
+    w = conn.dense_counts().astype(np.float64)  # [post, pre]
+    want = (w @ (sp * mo).T.astype(np.float64)).T
+    lib = L.lib()
+    for dt, fn in ((torch.int32, lib.fis_ev_spmv_i32),
+                   (torch.int64, lib.fis_ev_spmv_i64)):
+        crow = torch.from_numpy(conn.offsets).to("cuda", dt)
+        col = torch.from_numpy(conn.cols.astype(np.int64)).to("cuda", dt)
+        val = torch.from_numpy(conn.vals.astype(np.float32)).cuda()
+        spt, mot = torch.from_numpy(sp).cuda(), torch.from_numpy(mo).cuda()
+        out = torch.zeros(B, n, device="cuda")
+        p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        rc = fn(p(crow), p(col), p(val), p(spt), p(mot), p(out), n, B, 128,
+                ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+        assert rc == 0
+        np.testing.assert_array_equal(out.cpu().numpy(), want.astype(np.float32))
+    assert lib.fis_ev_spmv_abi() == 3
