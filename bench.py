@@ -1,0 +1,241 @@
+"""Benchmark: simulated trial-seconds per wall-second on one MI355X.
+
+Workloads (138,639 neurons, 15,091,983 edges, synthetic, seeded):
+  headline  sensory_b256: B = 256 trials, four sensory index sets driven at
+            piecewise-constant U(0, 300) Hz rates (10 ms segments), per-trial
+            m_out masks, per-trial Philox seeds; windows of 20 steps replayed
+            from one captured HIP graph
+  b1        plain_b1: B = 1, 20 neurons at 150 Hz -- the plain engine
+
+A bench "step" is one 0.1 ms model step of every trial.  ``value`` is
+B * steps * 1e-4 / wall seconds of the timed region (inputs and state
+resident in HBM).  The JSON line also carries the B = 1 rate, their ratio,
+the roofline of the dominant kernel (per-launch durations from HIP events
+on the launch stream) and a CPU baseline: the oracle's torch restatement of
+the reference step (csrmv via torch.sparse.mm) on this host's cores, over a
+bounded sample of the B = 1 workload.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for _p in (ROOT / "fly-in-silico_amd", ROOT / "oracle"):
+    if str(_p) not in sys.path:
+        sys.path.insert(0, str(_p))
+
+HBM_PEAK_GBS = 8000.0
+BETA = 26  # algorithmic bytes per neuron-trial-step of the dense update
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def timed(fn, sync):
+    sync()
+    t0 = time.perf_counter()
+    fn()
+    sync()
+    return time.perf_counter() - t0
+
+
+def kernel_times(runner, windows):
+    """Average durations (ms) of drive, update and scatter launches over
+    ``windows`` eager windows, bracketed by events on the launch stream."""
+    import torch
+
+    s = runner.sim
+    ev = {k: [] for k in ("drive", "update", "scatter")}
+    stream = torch.cuda.current_stream()
+    for _ in range(windows):
+        marks = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        marks[0].record(stream)
+        if runner.table is not None:
+            s.drive_table(runner.table, runner.window)
+        marks[1].record(stream)
+        s.update(runner.window)
+        marks[2].record(stream)
+        s.scatter()
+        marks[3].record(stream)
+        ev["drive"].append((marks[0], marks[1]))
+        ev["update"].append((marks[1], marks[2]))
+        ev["scatter"].append((marks[2], marks[3]))
+    torch.cuda.synchronize()
+    return {k: sum(a.elapsed_time(b) for a, b in v) / len(v)
+            for k, v in ev.items()}
+
+
+def scatter_bytes(conn, raster_rows):
+    """Algorithmic bytes of the scatter for given spikes: per spiking
+    (neuron, step) row 8 B offsets + 6 B per edge, plus an 8 B int32
+    read-modify-write per edge and spiking trial."""
+    import numpy as np
+
+    deg = conn.degree()
+    rows = 0
+    edges_trials = 0
+    for sp in raster_rows:  # bool [batch, n] per step
+        any_t = sp.any(axis=0)
+        rows += int((8 + 6 * deg[any_t]).sum())
+        edges_trials += int((deg[None, :] * sp).sum())
+    return rows + 8 * edges_trials
+
+
+def cpu_baseline(conn, wl, seconds):
+    """The oracle's torch restatement of the reference step on this host."""
+    import numpy as np
+    import torch
+
+    import ref_torch
+
+    cores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(cores)
+    eng = ref_torch.TorchRef(conn, 1, no_refractory=wl.drive_neuron)
+    rng = np.random.default_rng(0)
+    p = float(np.float32(wl.rate_hz[0, 0]) * np.float32(1e-4))
+    bits = np.zeros((1, conn.n), np.uint8)
+    for _ in range(5):
+        eng.step(bits)
+    steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        bits[:] = 0
+        kick = wl.drive_neuron[rng.random(wl.drive_neuron.size) < p]
+        bits[0, kick] = 1
+        eng.step(bits)
+        steps += 1
+    wall = time.perf_counter() - t0
+    return {"value": round(steps * 1e-4 / wall, 5), "unit": "trial-s/s",
+            "cores": cores, "kind": "port",
+            "sample": "%d steps of the B=1 workload (138,639 neurons, 20 "
+                      "neurons at 150 Hz), torch CPU csrmv restatement of "
+                      "the reference step, %.1f s" % (steps, wall),
+            "ms_per_step": round(1e3 * wall / steps, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--b1-steps", type=int, default=10_000)
+    ap.add_argument("--graph-windows", type=int, default=50)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=("all", "b256", "b1"), default="all")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the whole batch on "
+              "one MI355X (trials shard across GPUs with no exchange; see "
+              "DESIGN.md).  --gpus %d is not run." % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from flybrain_amd import connectome as C
+    from flybrain_amd import workloads as WL
+    from flybrain_amd.batch import BatchRunner
+    from flybrain_amd.device import DeviceNet
+
+    sync = torch.cuda.synchronize
+    t0 = time.perf_counter()
+    conn = C.full()
+    log("connectome %d neurons, %d edges, %s (%.1f s)"
+        % (conn.n, conn.n_edges, conn.fingerprint(), time.perf_counter() - t0))
+    net = DeviceNet(conn, "cuda")
+    out = {"metric": "sim trial-s per wall-s at 138k neurons, B=256 vs B=1",
+           "unit": "trial-s/s", "n_gpus": 1, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32+i32",
+           "data": "synthetic"}
+
+    res = {}
+    for name in (("b256", "b1") if args.only == "all" else (args.only,)):
+        if name == "b256":
+            wl = WL.sensory_b256(conn.n, seed=0, steps=args.warmup + args.steps)
+            steps = args.steps
+        else:
+            wl = WL.plain_b1(conn.n, seed=0, steps=args.warmup + args.b1_steps)
+            steps = args.b1_steps
+        r = BatchRunner(net, wl, counts=True)
+        r.capture(args.graph_windows)
+        r.advance(args.warmup)
+        wall = timed(lambda: r.advance(steps), sync)
+        spikes = int(r.counts().sum())
+        kt = kernel_times(r, 20)
+        res[name] = dict(wl=wl, runner=r, wall=wall, steps=steps,
+                         spikes=spikes, kt=kt,
+                         tps=wl.batch * steps * 1e-4 / wall)
+        log("%s: B=%d %d steps %.4f s  %.3f us/step  %.2f trial-s/s  "
+            "spikes/trial-step %.2f  kernels(ms) %s"
+            % (name, wl.batch, steps, wall, 1e6 * wall / steps,
+               res[name]["tps"],
+               spikes / (wl.batch * (steps + args.warmup)),
+               {k: round(v, 5) for k, v in kt.items()}))
+
+    head = res.get("b256") or res.get("b1")
+    wl = head["wl"]
+    n, B, W = conn.n, wl.batch, head["runner"].window
+    kt = head["kt"]
+    dom = max(("update", "scatter"), key=lambda k: kt[k])
+    if dom == "update":
+        alg = BETA * n * B * W
+    else:
+        r = head["runner"]
+        # spikes of the last window, from a short raster run
+        from flybrain_amd.batch import BatchRunner as _BR
+        rr = _BR(net, wl, counts=False, raster_len=W)
+        rr.advance(min(wl.steps, args.warmup + head["steps"]) // W * W - W)
+        rr.advance(W, use_graph=False)
+        rows = rr.sim.read_raster(rr.t - W, rr.t)
+        alg = scatter_bytes(conn, rows)
+        del rr
+    achieved = alg / (kt[dom] * 1e-3) / 1e9
+    out.update({
+        "value": round(head["tps"], 3),
+        "steps": head["steps"], "warmup": args.warmup,
+        "ms_per_step": round(1e3 * head["wall"] / head["steps"], 6),
+        "config": {"workload": wl.name, "neurons": n,
+                   "edges": conn.n_edges, "batch": B,
+                   "window_steps": W, "graph_windows": args.graph_windows,
+                   "connectome": conn.fingerprint()},
+        "roofline": {"bound": "hbm", "kernel": "k_update" if dom == "update"
+                     else "k_scatter", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "algorithmic_bytes_per_launch": int(alg),
+                     "avg_launch_ms": round(kt[dom], 5)},
+        "kernels_ms": {k: round(v, 5) for k, v in kt.items()},
+    })
+    if "b1" in res and "b256" in res:
+        out["b1"] = {"value": round(res["b1"]["tps"], 3),
+                     "steps": res["b1"]["steps"],
+                     "ms_per_step": round(1e3 * res["b1"]["wall"]
+                                          / res["b1"]["steps"], 6),
+                     "kernels_ms": {k: round(v, 5)
+                                    for k, v in res["b1"]["kt"].items()}}
+        out["ratio_b256_over_b1"] = round(res["b256"]["tps"]
+                                          / res["b1"]["tps"], 3)
+    for name, rr in res.items():
+        out.setdefault("spikes_per_trial_step", {})[name] = round(
+            rr["spikes"] / (rr["wl"].batch * (rr["steps"] + args.warmup)),
+            3)
+    if not args.no_cpu:
+        b1 = WL.plain_b1(conn.n, seed=0)
+        out["cpu_baseline"] = cpu_baseline(conn, b1, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -82,9 +82,8 @@ __device__ __forceinline__ uint32_t window_bits(int64_t t0, int W,
   return bits;
 }
 
-__device__ __forceinline__ int64_t n_waves_of(const fis_sim& s) {
-  const int64_t nb = (int64_t)s.n * s.bstride;
-  return (nb + kWave - 1) / kWave;
+__device__ __forceinline__ int n_waves_of(const fis_sim& s) {
+  return (s.n * s.bstride + kWave - 1) >> 6;
 }
 
 // ------------------------------------------------------------- drive
@@ -92,11 +91,11 @@ __device__ __forceinline__ int64_t n_waves_of(const fis_sim& s) {
 __global__ __launch_bounds__(kBlock) void k_drive_rates(fis_sim s,
                                                         const float* rates,
                                                         int W) {
-  const int64_t nb = (int64_t)s.n * s.bstride;
-  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int nb = s.n * s.bstride;
+  const int idx = (int)blockIdx.x * kBlock + (int)threadIdx.x;
   if (idx >= nb) return;
-  const int64_t i = idx / s.bstride;
-  const int b = (int)(idx - i * s.bstride);
+  const int i = idx / s.bstride;
+  const int b = idx - i * s.bstride;
   uint32_t bits = 0;
   if (b < s.batch) {
     const float p = rates[(int64_t)b * s.n + i] * s.prob_scale;
@@ -113,8 +112,8 @@ __global__ __launch_bounds__(kBlock) void k_drive_table(
   const int j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= n_entries) return;
   const fis_drive_entry e = entries[j];
-  const int64_t i = (int64_t)e.idx / s.bstride;
-  const int b = (int)((int64_t)e.idx - i * s.bstride);
+  const int i = (int)e.idx / s.bstride;
+  const int b = (int)e.idx - i * s.bstride;
   const float ps = s.prob_scale;
   const uint32_t bits = window_bits(
       *s.clock, W, (uint32_t)i, s.trial_ids[b], s.seeds[b], [&](int64_t t) {
@@ -122,7 +121,9 @@ __global__ __launch_bounds__(kBlock) void k_drive_table(
         if (seg >= n_seg) seg = n_seg - 1;
         return rate_hz[seg * n_groups + e.group] * ps;
       });
-  s.sbits[e.idx] = bits;
+  // entries of neighbouring neuron-trials may sit in other workgroups: set
+  // the bits with an atomic so no two workgroups store into one line
+  if (bits) atomicOr(&s.sbits[e.idx], bits);
 }
 
 __global__ __launch_bounds__(kBlock) void k_drive_tape(fis_sim s,
@@ -140,58 +141,99 @@ __global__ __launch_bounds__(kBlock) void k_drive_tape(fis_sim s,
 
 // ------------------------------------------------------------- update
 
+// Lane masks.  A ballot lands in a scalar register pair; it is used here
+// only by scalar control flow (is any lane set?).  Every mask that is
+// stored to memory is built in vector registers instead, by OR-reduction
+// across lanes, and positions come from a vector prefix scan.  Measured on
+// gfx950: a ballot copied into a vector register for a store
+// (v_mov_b64 v, s[pair]) intermittently arrived with lanes 32..63 cleared,
+// while scalar tests of the same ballot were always right.
+__device__ __forceinline__ bool any_lane(bool p) {
+  uint64_t m = __ballot(p);
+  asm volatile("s_mov_b64 %0, %1" : "=s"(m) : "s"(m));
+  return m != 0;
+}
+
+// OR of `x` over aligned groups of `width` lanes (width a power of two <= 64);
+// every lane of a group receives the group's value.
+__device__ __forceinline__ uint64_t group_or(uint64_t x, int width) {
+  uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+  for (int o = 1; o < width; o <<= 1) {
+    lo |= (uint32_t)__shfl_xor((int)lo, o);
+    hi |= (uint32_t)__shfl_xor((int)hi, o);
+  }
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Exclusive prefix count of `p` over the wave, and the wave total.
+__device__ __forceinline__ int prefix_count(bool p, int lane, int& total) {
+  const int x = p ? 1 : 0;
+  int inc = x;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int y = __shfl_up(inc, o);
+    inc += (lane >= o) ? y : 0;
+  }
+  total = __shfl(inc, kWave - 1);
+  return inc - x;
+}
+
 // Spike compaction of one step: the raster row (optional) and the events
-// of transmitted spikes, appended to this wave's segment.
-__device__ __forceinline__ void emit(const fis_sim& s, bool fired,
-                                     bool sends, int lane, int64_t idx,
-                                     int64_t wave, int64_t t, int slot,
+// of transmitted spikes, appended to this wave's segment.  `pre` and `tb`
+// are the lane's neuron and trial (computed once per thread).  Stores are
+// made by every lane (identical values, or a sink slot for lanes with
+// nothing to write), so no branch narrows the wave around them.
+__device__ __forceinline__ void emit(const fis_sim& s, bool fired, bool sends,
+                                     int lane, int pre, int tb, int wave,
+                                     int nblk, int rrow, int slot,
                                      fis_event* evs, int& nev) {
   const int bs = s.bstride;
   if (s.raster) {
-    const uint64_t all = __ballot(fired);
-    if (lane == 0) s.raster[(t % s.raster_len) * n_waves_of(s) + wave] = all;
+    const uint64_t all = group_or(fired ? (1ull << lane) : 0ull, kWave);
+    s.raster[((int64_t)rrow * nblk + (wave >> 2)) * FIS_LINE_WORDS +
+             (wave & 3)] = all;
   }
-  const uint64_t bal = __ballot(sends);
-  if (!bal) return;
+  if (!any_lane(sends)) return;
   if (bs >= kWave) {
-    if (lane == 0) {
-      fis_event e;
-      e.pre = (int32_t)(idx / bs);
-      e.slot = (uint16_t)slot;
-      e.tb = (uint16_t)(idx % bs);
-      e.mask = bal;
-      evs[nev] = e;
-    }
+    fis_event e;
+    e.pre = pre;
+    e.slot = (uint16_t)slot;
+    e.tb = (uint16_t)(tb - lane);  // trial of lane 0
+    e.mask = group_or(sends ? (1ull << lane) : 0ull, kWave);
+    evs[nev] = e;  // every lane writes the same record
     nev += 1;
   } else {
     // 64/bs neurons per wave; the first lane of each neuron leads
-    const int j0 = lane - (lane % bs);
-    const uint64_t sub = (bal >> j0) & ((1ull << bs) - 1ull);
-    const bool lead = (lane % bs) == 0 && sub != 0;
-    const uint64_t lb = __ballot(lead);
-    if (lead) {
-      const int pos = __popcll(lb & ((1ull << lane) - 1ull));
-      fis_event e;
-      e.pre = (int32_t)(idx / bs);
-      e.slot = (uint16_t)slot;
-      e.tb = 0;
-      e.mask = sub;
-      evs[nev + pos] = e;
-    }
-    nev += __popcll(lb);
+    const uint64_t sub = group_or(sends ? (1ull << tb) : 0ull, bs);
+    const bool lead = tb == 0 && sub != 0;
+    int total;
+    const int pos = prefix_count(lead, lane, total);
+    // lanes that do not lead write the segment's last slot (a sink)
+    const int at = lead ? nev + pos : s.ev_cap - 1;
+    fis_event e;
+    e.pre = pre;
+    e.slot = (uint16_t)slot;
+    e.tb = 0;
+    e.mask = sub;
+    evs[at] = e;
+    nev += total;
   }
 }
 
 // W steps per neuron-trial.  WMAX is the unrolled bound (1 or 20).
+// Index arithmetic is 32-bit: n * bstride < 2^31 is checked on the host.
 template <int WMAX>
 __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
-  const int64_t nb = (int64_t)s.n * s.bstride;
-  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int64_t wave = idx >> 6;
+  const int bs = s.bstride;
+  const int nb = s.n * bs;
+  const int idx = (int)blockIdx.x * kBlock + (int)threadIdx.x;
+  const int lane = (int)threadIdx.x & (kWave - 1);
+  const int wave = idx >> 6;
+  const int nblk = (int)gridDim.x;
   const bool act = idx < nb;
   const int64_t t0 = *s.clock;
-  const int bs = s.bstride;
+  const int pre = idx / bs;       // neuron of this lane
+  const int tb = idx - pre * bs;  // trial of this lane
 
   float v = 0.0f, g = 0.0f, vh = 0.0f;
   uint32_t st = 0, fl = FIS_FIRE_OFF, sb = 0;
@@ -219,81 +261,86 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
   int r = (int)(st & FIS_ST_RMAX);
   float sp = (st & FIS_ST_SPIKE) ? 1.0f : 0.0f;
 
-  // the W ring reads are independent of the recurrence: issue them first
+  // ring reads run PREFETCH steps ahead of the recurrence (they do not
+  // depend on it); the step loop itself stays rolled, which keeps the
+  // per-step state out of scalar registers
+  constexpr int PREFETCH = WMAX < 4 ? WMAX : 4;
   int32_t* rp = s.ring + idx;
-  const int slot0 = (int)((t0 + 1) % FIS_RING_LEN);
-  int32_t in[WMAX];
+  int slot = (int)((t0 + 1) % FIS_RING_LEN);
+  int32_t q[PREFETCH];
+  int pslot = slot;
 #pragma unroll
-  for (int k = 0; k < WMAX; ++k) {
-    int slot = slot0 + k;
-    slot -= (slot >= FIS_RING_LEN) ? FIS_RING_LEN : 0;
-    in[k] = (k < W && act) ? rp[(int64_t)slot * nb] : 0;
+  for (int j = 0; j < PREFETCH; ++j) {
+    q[j] = (j < W && act) ? rp[(int64_t)pslot * nb] : 0;
+    pslot = (pslot + 1 == FIS_RING_LEN) ? 0 : pslot + 1;
   }
 
-  fis_event* evs = s.events + wave * s.ev_cap;
+  fis_event* evs = s.events + (int64_t)wave * s.ev_cap;
   int nev = 0;  // wave-uniform
   uint32_t nspk = 0;
+  int rrow = s.raster ? (int)(t0 % s.raster_len) : 0;
 
+#pragma unroll 1
+  for (int k = 0; k < W; ++k) {
+    const int32_t I = q[0];
 #pragma unroll
-  for (int k = 0; k < WMAX; ++k) {
-    if (k < W) {
-      int slot = slot0 + k;
-      slot -= (slot >= FIS_RING_LEN) ? FIS_RING_LEN : 0;
-      const int32_t I = in[k];
-      if (act && I != 0) rp[(int64_t)slot * nb] = 0;
-
-      // refractory counter: r = (r + 1) * (1 - s), saturating
-      r = (sp != 0.0f) ? 0 : (r < FIS_ST_RMAX ? r + 1 : FIS_ST_RMAX);
-      const float gate = (r >= R) ? 1.0f : 0.0f;
-
-      // conductance with the delayed input (sim.py:927-929)
-      const float rec = s.w_scale * (float)I;
-      const float del = rec * m_in;
-      const float gd = g * s.a_syn;
-      const float dg = del * gate;
-      const float g_new = gd + dg;
-
-      // membrane with the OLD conductance (sim.py:937-939)
-      const float stim = ((sb >> k) & 1u) ? kick1 : kick0;
-      float vv = v + stim;
-      const float t1 = vv - s.v_rest;
-      const float t2 = g - t1;
-      const float t3 = s.a_mem * t2;
-      vv = vv + t3;
-      vv = vv * m_hold;  // hold before threshold (sim.py:944)
-      vv = vv + v_hold;
-      float spk = (vv > s.v_th) ? 1.0f : 0.0f;
-      spk = spk * m_fire;
-      const float t4 = vv - s.v_reset;  // reset by the masked spike
-      const float t5 = t4 * spk;
-      vv = vv - t5;
-      const float gs = g_new * spk;
-      g = g_new - gs;
-      v = vv;
-      sp = spk;
-
-      const bool fired = act && spk != 0.0f;
-      nspk += fired ? 1u : 0u;
-      emit(s, fired, fired && transmit, lane, idx, wave, t0 + k, slot, evs,
-           nev);
+    for (int j = 0; j + 1 < PREFETCH; ++j) q[j] = q[j + 1];
+    if (PREFETCH > 0) {
+      q[PREFETCH - 1] = (k + PREFETCH < W && act) ? rp[(int64_t)pslot * nb] : 0;
+      pslot = (pslot + 1 == FIS_RING_LEN) ? 0 : pslot + 1;
     }
+    if (act && I != 0) rp[(int64_t)slot * nb] = 0;
+
+    // refractory counter: r = (r + 1) * (1 - s), saturating
+    r = (sp != 0.0f) ? 0 : (r < FIS_ST_RMAX ? r + 1 : FIS_ST_RMAX);
+    const float gate = (r >= R) ? 1.0f : 0.0f;
+
+    // conductance with the delayed input (sim.py:927-929)
+    const float rec = s.w_scale * (float)I;
+    const float del = rec * m_in;
+    const float gd = g * s.a_syn;
+    const float dg = del * gate;
+    const float g_new = gd + dg;
+
+    // membrane with the OLD conductance (sim.py:937-939)
+    const float stim = ((sb >> k) & 1u) ? kick1 : kick0;
+    float vv = v + stim;
+    const float t1 = vv - s.v_rest;
+    const float t2 = g - t1;
+    const float t3 = s.a_mem * t2;
+    vv = vv + t3;
+    vv = vv * m_hold;  // hold before threshold (sim.py:944)
+    vv = vv + v_hold;
+    float spk = (vv > s.v_th) ? 1.0f : 0.0f;
+    spk = spk * m_fire;
+    const float t4 = vv - s.v_reset;  // reset by the masked spike
+    const float t5 = t4 * spk;
+    vv = vv - t5;
+    const float gs = g_new * spk;
+    g = g_new - gs;
+    v = vv;
+    sp = spk;
+
+    const bool fired = act && spk != 0.0f;
+    nspk += fired ? 1u : 0u;
+    emit(s, fired, fired && transmit, lane, pre, tb, wave, nblk, rrow, slot,
+         evs, nev);
+    rrow = (rrow + 1 == s.raster_len) ? 0 : rrow + 1;
+    slot = (slot + 1 == FIS_RING_LEN) ? 0 : slot + 1;
   }
 
   if (act) {
     s.v[idx] = v;
     s.g[idx] = g;
     s.st[idx] = (uint8_t)((sp != 0.0f ? FIS_ST_SPIKE : 0u) | (uint32_t)r);
-    if (s.spikes_out) {
-      const int64_t i = idx / bs;
-      const int b = (int)(idx - i * bs);
-      if (b < s.batch) s.spikes_out[(int64_t)b * s.n + i] = sp;
-    }
+    if (s.spikes_out) s.spikes_out[idx] = sp;
     if (nspk) {
       if (s.counts) s.counts[idx] += nspk;
-      if (s.pop_counts) atomicAdd(&s.pop_counts[idx / bs], nspk);
+      if (s.pop_counts) atomicAdd(&s.pop_counts[pre], nspk);
     }
   }
-  if (lane == 0 && act) s.ev_count[wave] = nev;
+  if (lane == 0 && act)
+    s.ev_count[(int64_t)blockIdx.x * FIS_LINE_INTS + (wave & 3)] = nev;
 
   // the last block to finish advances the clock for the next window
   __syncthreads();
@@ -315,12 +362,13 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
 __global__ __launch_bounds__(kBlock) void k_scatter(fis_sim s) {
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t nw = n_waves_of(s);
-  const int64_t nb = (int64_t)s.n * s.bstride;
+  const int64_t nb = (int64_t)s.n * s.bstride;  // ring slot stride
   const int bs = s.bstride;
   const int64_t stride = (int64_t)gridDim.x * (kBlock / kWave);
   for (int64_t w = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6; w < nw;
        w += stride) {
-    const int cnt = __builtin_amdgcn_readfirstlane(s.ev_count[w]);
+    const int cnt = __builtin_amdgcn_readfirstlane(
+        s.ev_count[(w >> 2) * FIS_LINE_INTS + (w & 3)]);
     if (cnt == 0) continue;
     const fis_event* evs = s.events + w * s.ev_cap;
     for (int e = 0; e < cnt; ++e) {
@@ -345,19 +393,19 @@ __global__ __launch_bounds__(kBlock) void k_scatter(fis_sim s) {
 
 // Current spikes (st bit 7, transmitted) into one ring slot.
 __global__ __launch_bounds__(kBlock) void k_seed_ring(fis_sim s, int slot) {
-  const int64_t nb = (int64_t)s.n * s.bstride;
-  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int nb = s.n * s.bstride;
+  const int idx = (int)blockIdx.x * kBlock + (int)threadIdx.x;
   const int lane = threadIdx.x & (kWave - 1);
   const bool hot = idx < nb && (s.st[idx] & FIS_ST_SPIKE) &&
                    !(s.fl[idx] & FIS_OUT_OFF);
   uint64_t bal = __ballot(hot);
-  const int64_t base_idx = idx - lane;
+  const int base_idx = idx - lane;
   while (bal) {
     const int j = __ffsll((long long)bal) - 1;
     bal &= bal - 1ull;
-    const int64_t src = base_idx + j;
-    const int64_t pre = src / s.bstride;
-    const int b = (int)(src - pre * s.bstride);
+    const int src = base_idx + j;
+    const int pre = src / s.bstride;
+    const int b = src - pre * s.bstride;
     int32_t* base = s.ring + (int64_t)slot * nb + b;
     for (int64_t k = s.offsets[pre] + lane; k < s.offsets[pre + 1]; k += kWave)
       __hip_atomic_fetch_add(base + (int64_t)s.cols[k] * s.bstride,
@@ -403,13 +451,19 @@ int check_sim(const fis_sim* s) {
   if (!s || s->n <= 0 || s->batch <= 0 || !valid_stride(s->bstride) ||
       s->batch > s->bstride || s->bstride > 65535 + 1)
     return (int)hipErrorInvalidValue;
-  const int need = FIS_MAX_WINDOW * (s->bstride >= kWave ? 1 : kWave / s->bstride);
-  if (s->ev_cap < need) return (int)hipErrorInvalidValue;
+  // events of a full window plus one sink slot per wave segment
+  const int need =
+      FIS_MAX_WINDOW * (s->bstride >= kWave ? 1 : kWave / s->bstride) + 1;
+  if (s->ev_cap < need || s->ev_cap % FIS_EVENT_ALIGN != 0)
+    return (int)hipErrorInvalidValue;
   if (!s->offsets || !s->cols || !s->vals || !s->v || !s->g || !s->st ||
       !s->fl || !s->ring || !s->sbits || !s->events || !s->ev_count ||
       !s->clock || !s->done || !s->seeds || !s->trial_ids)
     return (int)hipErrorInvalidValue;
-  if (s->raster && s->raster_len <= 0) return (int)hipErrorInvalidValue;
+  if (s->raster && s->raster_len < FIS_MAX_WINDOW)
+    return (int)hipErrorInvalidValue;
+  if ((int64_t)s->n * s->bstride >= ((int64_t)1 << 31) - kBlock)
+    return (int)hipErrorInvalidValue;
   return 0;
 }
 
@@ -436,6 +490,11 @@ int64_t fis_struct_layout(int what) {
     case 6: return (int64_t)sizeof(fis_drive_entry);
     default: return -1;
   }
+}
+
+int64_t fis_n_blocks(int32_t n, int32_t bstride) {
+  const int64_t nb = (int64_t)n * bstride;
+  return (nb + kBlock - 1) / kBlock;
 }
 
 int64_t fis_n_waves(int32_t n, int32_t bstride) {

@@ -25,6 +25,7 @@ ABI_VERSION = 1
 MAX_WINDOW = 20
 RING_LEN = 20
 FIRE_OFF, OUT_OFF, IN_OFF, HOLD, NOREFRAC = 0x01, 0x02, 0x04, 0x08, 0x10
+BLOCK_WAVES, LINE_WORDS, LINE_INTS, EVENT_ALIGN = 4, 16, 32, 8
 ST_SPIKE, ST_RMAX = 0x80, 127
 
 HIPCC_FLAGS = [
@@ -113,6 +114,7 @@ def lib():
         "fis_ev_spmv_abi": ([], I),
         "fis_struct_layout": ([I], I64),
         "fis_n_waves": ([ctypes.c_int32, ctypes.c_int32], I64),
+        "fis_n_blocks": ([ctypes.c_int32, ctypes.c_int32], I64),
         "fis_ev_spmv_i32": ([P] * 6 + [I, I, I, P], I),
         "fis_ev_spmv_i64": ([P] * 6 + [I, I, I, P], I),
         "fis_drive_rates": ([sims, P, I, P], I),

@@ -49,7 +49,10 @@ class DeviceNet:
     def __init__(self, conn, device="cuda"):
         self.conn = conn
         self.n = conn.n
-        self.device = torch.device(device)
+        dev = torch.device(device)
+        if dev.type == "cuda" and dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
+        self.device = dev
         self.offsets = torch.from_numpy(conn.offsets).to(self.device)
         self.cols = torch.from_numpy(conn.cols).to(self.device)
         self.vals = torch.from_numpy(conn.vals).to(self.device)
@@ -76,7 +79,11 @@ class DeviceSim:
         self.bstride = bs = stride_for(B)
         self.nb = nb = n * bs
         self.n_waves = int(self.lib.fis_n_waves(n, bs))
-        self.ev_cap = L.MAX_WINDOW * (1 if bs >= 64 else 64 // bs)
+        self.n_blocks = int(self.lib.fis_n_blocks(n, bs))
+        # a window's events plus one sink slot per wave segment, rounded up
+        # so every segment starts on its own 128-byte line
+        cap = L.MAX_WINDOW * (1 if bs >= 64 else 64 // bs) + 1
+        self.ev_cap = (cap + L.EVENT_ALIGN - 1) // L.EVENT_ALIGN * L.EVENT_ALIGN
         dev = net.device
         self.device = dev
         z = dict(device=dev)
@@ -84,12 +91,14 @@ class DeviceSim:
         self.g = torch.zeros(nb, dtype=torch.float32, **z)
         self.st = torch.zeros(nb, dtype=torch.uint8, **z)
         self.fl = torch.zeros(nb, dtype=torch.uint8, **z)
-        self.vhold = None
+        # allocated up front: a captured graph holds this pointer
+        self.vhold = torch.zeros(nb, dtype=torch.float32, **z)
         self.ring = torch.zeros(L.RING_LEN * nb, dtype=torch.int32, **z)
         self.sbits = torch.zeros(nb, dtype=torch.int32, **z)
         self.events = torch.zeros(self.n_waves * self.ev_cap * 2,
                                   dtype=torch.int64, **z)
-        self.ev_count = torch.zeros(self.n_waves, dtype=torch.int32, **z)
+        self.ev_count = torch.zeros(self.n_blocks * L.LINE_INTS,
+                                    dtype=torch.int32, **z)
         self.clock = torch.zeros(1, dtype=torch.int64, **z)
         self.done = torch.zeros(1, dtype=torch.int32, **z)
         sd = np.zeros(bs, np.uint64)
@@ -101,22 +110,34 @@ class DeviceSim:
                                                                   np.uint32)
         self.trial_ids_host = ti
         self.trial_ids = torch.from_numpy(ti.view(np.int32)).to(dev)
-        self.spikes_out = (torch.zeros(B * n, dtype=torch.float32, **z)
+        self.spikes_out = (torch.zeros(nb, dtype=torch.float32, **z)
                            if spikes_out else None)
         self.counts = torch.zeros(nb, dtype=torch.int32, **z) if counts else None
         self.pop_counts = (torch.zeros(n, dtype=torch.int32, **z)
                            if pop_counts else None)
         self.raster_len = int(raster_len)
-        self.raster = (torch.zeros(self.raster_len * self.n_waves,
-                                   dtype=torch.int64, **z)
+        if self.raster_len and self.raster_len < L.MAX_WINDOW:
+            raise ValueError("raster_len must be 0 or >= %d" % L.MAX_WINDOW)
+        self.raster = (torch.zeros(self.raster_len * self.n_blocks
+                                   * L.LINE_WORDS, dtype=torch.int64, **z)
                        if raster_len else None)
         self.t = 0  # host mirror of the device clock
 
-        # refractory period per neuron-trial and padding trials
-        fl = self.fl.view(n, bs)
+        # Host mirrors of the host-written per-neuron-trial arrays.  Changes
+        # are made here and sent down as one contiguous copy: scattered
+        # index writes by many workgroups could share 128-byte lines.
+        self.fl_host = np.zeros((n, bs), np.uint8)
         if bs > B:
-            fl[:, B:] = L.FIRE_OFF
-        self.set_norefrac(no_refractory)
+            self.fl_host[:, B:] = L.FIRE_OFF
+        self.vhold_host = np.zeros((n, bs), np.float32)
+        self.version = 0  # bumped whenever the kernel struct changes
+        st0 = np.full((n, bs), int(self.c["refrac_steps"]), np.uint8)
+        free = self._neurons(no_refractory)
+        if free.size:
+            self.fl_host[free, :B] |= L.NOREFRAC
+            st0[free, :B] = 0
+        self.st.copy_(torch.from_numpy(st0.reshape(-1)))
+        self._push_flags()
         self._sync_struct()
 
     # ------------------------------------------------------------ struct
@@ -142,6 +163,7 @@ class DeviceSim:
         s.raster_len = self.raster_len
         self.struct = s
         self._sp = ctypes.byref(s)
+        self.version = getattr(self, "version", 0) + 1
 
     # ------------------------------------------------------------ flags
     def _trials(self, trials):
@@ -158,18 +180,18 @@ class DeviceSim:
             raise IndexError("neuron index outside [0, %d)" % self.n)
         return a
 
+    def _push_flags(self):
+        self.fl.copy_(torch.from_numpy(self.fl_host.reshape(-1)))
+
     def set_flag(self, bit, neurons, trials=None):
         """Clear ``bit`` on the given trials, then set it on ``neurons``."""
         tr = self._trials(trials)
         ne = self._neurons(neurons)
-        fl = self.fl.view(self.n, self.bstride)
-        tt = torch.as_tensor(tr, device=self.device)
-        cur = fl[:, tt]
-        fl[:, tt] = cur & (0xFF ^ bit)
+        f = self.fl_host
+        f[:, tr] &= np.uint8(0xFF ^ bit)
         if ne.size:
-            nn = torch.as_tensor(ne, device=self.device)
-            sub = fl[nn][:, tt]
-            fl[nn.unsqueeze(1), tt.unsqueeze(0)] = sub | bit
+            f[np.ix_(ne, tr)] |= np.uint8(bit)
+        self._push_flags()
 
     def set_flag_pairs(self, bit, neurons, trials):
         """Set ``bit`` on explicit (neuron, trial) pairs (other entries kept)."""
@@ -180,37 +202,38 @@ class DeviceSim:
         if tr.size and (tr.min() < 0 or tr.max() >= self.batch):
             raise IndexError("trial index outside [0, %d)" % self.batch)
         if ne.size:
-            idx = torch.as_tensor(ne * self.bstride + tr, device=self.device)
-            self.fl[idx] = self.fl[idx] | bit
+            self.fl_host[ne, tr] |= np.uint8(bit)
+            self._push_flags()
 
-    def set_norefrac(self, neurons, trials=None):
-        """Refractory period 0 for ``neurons`` (sim.py:371-381)."""
-        self.set_flag(L.NOREFRAC, neurons, trials)
-        # the initial counter value is refrac_steps (0 for exempt neurons),
-        # as Engine.__init__ sets refrac = refrac_steps (sim.py:388)
-        R = int(self.c["refrac_steps"])
-        fl = self.fl.view(self.n, self.bstride)
-        st = self.st.view(self.n, self.bstride)
-        fresh = (st == 0) | (st == R)
-        if self.t == 0:
-            free = (fl & L.NOREFRAC) != 0
-            st[fresh & free] = 0
-            st[fresh & ~free] = R
+    def exempt_pairs(self, neurons, trials):
+        """Refractory period 0 for (neuron, trial) pairs, before the first
+        step: the counter starts at refrac_steps = 0 for them, as
+        Engine.__init__ sets refrac = refrac_steps (sim.py:371-388)."""
+        if self.t != 0:
+            raise RuntimeError("refractory exemptions are set before stepping")
+        ne = self._neurons(neurons)
+        tr = np.asarray(trials, dtype=np.int64).reshape(-1)
+        if ne.size:
+            self.fl_host[ne, tr] |= np.uint8(L.NOREFRAC)
+            self._push_flags()
+            st = self.st.cpu().numpy().reshape(self.n, self.bstride)
+            st[ne, tr] = 0
+            self.st.copy_(torch.from_numpy(st.reshape(-1)))
 
     def set_hold(self, neurons, v_clamp, trials=None):
-        self.set_flag(L.HOLD, neurons, trials)
-        if self.vhold is None:
-            self.vhold = torch.zeros(self.nb, dtype=torch.float32,
-                                     device=self.device)
-            self._sync_struct()
         tr = self._trials(trials)
         ne = self._neurons(neurons)
-        vh = self.vhold.view(self.n, self.bstride)
-        tt = torch.as_tensor(tr, device=self.device)
-        vh[:, tt] = 0.0
+        self.vhold_host[:, tr] = 0.0
         if ne.size:
-            nn = torch.as_tensor(ne, device=self.device)
-            vh[nn.unsqueeze(1), tt.unsqueeze(0)] = float(np.float32(v_clamp))
+            self.vhold_host[np.ix_(ne, tr)] = np.float32(v_clamp)
+        self.vhold.copy_(torch.from_numpy(self.vhold_host.reshape(-1)))
+        self.set_flag(L.HOLD, neurons, trials)
+
+    def write_trials(self, buf, arr):
+        """Write [batch, n] values into a trial-minor buffer as one copy."""
+        host = buf.cpu().numpy().reshape(self.n, self.bstride).copy()
+        host[:, :self.batch] = np.asarray(arr).reshape(self.batch, self.n).T
+        buf.copy_(torch.from_numpy(host.reshape(-1)))
 
     # ------------------------------------------------------------ launches
     def drive_rates(self, rates, W=1):
@@ -288,9 +311,12 @@ class DeviceSim:
         if t_to - t_from > self.raster_len or t_from < self.t - self.raster_len:
             raise ValueError("raster rows [%d, %d) no longer held"
                              % (t_from, t_to))
-        rows = [t % self.raster_len for t in range(t_from, t_to)]
-        r = self.raster.view(self.raster_len, self.n_waves)[rows].cpu().numpy()
-        bits = np.unpackbits(r.view(np.uint8), axis=1, bitorder="little")
+        rows = np.array([t % self.raster_len for t in range(t_from, t_to)])
+        full = self.raster.cpu().numpy().reshape(
+            self.raster_len, self.n_blocks, L.LINE_WORDS)
+        words = full[rows][:, :, :L.BLOCK_WAVES].reshape(len(rows), -1)
+        words = np.ascontiguousarray(words[:, :self.n_waves])
+        bits = np.unpackbits(words.view(np.uint8), axis=1, bitorder="little")
         bits = bits[:, :self.nb].reshape(len(rows), self.n, self.bstride)
         return bits[:, :, :self.batch].transpose(0, 2, 1).astype(bool)
 

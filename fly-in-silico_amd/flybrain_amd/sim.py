@@ -209,6 +209,7 @@ class Engine:
                 % ex["max_row_abs_sum"])
         self.conn = conn
         self._net = DeviceNet(conn, dev)
+        self.device = dev = self._net.device
         self.tau_syn_f = DT / p["tauSyn"]
         self.tau_mem_f = DT / p["tauMem"]
         self.prob_scale = DT / 1000.0
@@ -249,7 +250,7 @@ class Engine:
 
     @property
     def spikes(self):
-        return self._sim.spikes_out.view(self.batch, self.n)
+        return self._sim.view(self._sim.spikes_out)
 
     @property
     def refrac(self):
@@ -262,14 +263,17 @@ class Engine:
 
     @property
     def delay_buffer(self):
-        """The reference's [B, 19, n] ring of synaptic input in mV."""
-        T = self._sim.t
-        out = torch.zeros(self.batch, self.buf_len, self.n,
-                          dtype=torch.float32, device=self.device)
+        """The reference's [B, 19, n] ring of synaptic input in mV
+        (assembled on the host from one copy of the int32 ring)."""
+        s = self._sim
+        T = s.t
+        ring = s.ring.cpu().numpy().reshape(L.RING_LEN, s.n, s.bstride)
+        w = np.float32(self.w_scale)
+        out = np.zeros((self.batch, self.buf_len, self.n), np.float32)
         for u in range(max(0, T - self.buf_len), T):
-            out[:, u % self.buf_len, :] = float(np.float32(self.w_scale)) * \
-                self._sim.rec_slot(u).to(torch.float32)
-        return out
+            I = ring[u % L.RING_LEN, :, :self.batch].T
+            out[:, u % self.buf_len, :] = w * I.astype(np.float32)
+        return torch.from_numpy(out)
 
     def _mask(self, bit):
         f = self._sim.view(self._sim.fl)
@@ -293,8 +297,6 @@ class Engine:
 
     @property
     def v_hold(self):
-        if self._sim.vhold is None:
-            return torch.zeros(self.batch, self.n, device=self.device)
         return self._sim.view(self._sim.vhold) * (1.0 - self.m_hold)
 
     # --- road report ------------------------------------------------------
@@ -397,6 +399,7 @@ class Engine:
             raise
         self._graph = g
         self._graphs = [g]
+        self._graph_version = self._sim.version
         return True
 
     def _device_snapshot(self):
@@ -421,6 +424,10 @@ class Engine:
         if tuple(r.shape) != (self.batch, self.n):
             raise ValueError("rates must be [%d, %d], got %s"
                              % (self.batch, self.n, tuple(r.shape)))
+        if self._graph is not None and self._graph_version != self._sim.version:
+            # the kernel arguments changed since capture: capture again
+            self._graph = self._graphs = None
+            self.use_graphs()
         if self._graph is not None:
             if r.data_ptr() != self._rates.data_ptr():
                 self._rates.copy_(r)
@@ -490,54 +497,49 @@ class Engine:
                                  % (int(state["ptr"]), T))
             done.append("ptr")
 
-        def put(buf, arr, dtype):
-            x = torch.as_tensor(np.asarray(arr), dtype=dtype,
-                                device=self.device)
-            s.view(buf).copy_(x.reshape(B, n))
-
         if "v" not in drop and "v" in state:
-            put(s.v, state["v"], torch.float32)
+            s.write_trials(s.v, np.asarray(state["v"], np.float32))
             done.append("v")
         if "conductance" not in drop and "conductance" in state:
-            put(s.g, state["conductance"], torch.float32)
+            s.write_trials(s.g, np.asarray(state["conductance"], np.float32))
             done.append("conductance")
-        st = s.view(s.st).clone()
+        st = s.st.cpu().numpy().reshape(n, bs).copy()
         if "refrac" not in drop and "refrac" in state:
-            r = np.minimum(np.asarray(state["refrac"], np.float64), L.ST_RMAX)
-            rr = torch.as_tensor(r.astype(np.uint8), device=self.device)
-            st = (st & L.ST_SPIKE) | rr
+            r = np.minimum(np.asarray(state["refrac"], np.float64),
+                           L.ST_RMAX).astype(np.uint8).reshape(B, n).T
+            st[:, :B] = (st[:, :B] & L.ST_SPIKE) | r
             done.append("refrac")
         if "spikes" not in drop and "spikes" in state:
-            sp = torch.as_tensor(np.asarray(state["spikes"]) != 0,
-                                 device=self.device)
-            st = (st & L.ST_RMAX) | (sp.to(torch.uint8) * L.ST_SPIKE)
+            sp = (np.asarray(state["spikes"]).reshape(B, n).T != 0)
+            st[:, :B] = (st[:, :B] & L.ST_RMAX) | (sp.astype(np.uint8)
+                                                  * L.ST_SPIKE)
             done.append("spikes")
-        s.view(s.st).copy_(st)
+        s.st.copy_(torch.from_numpy(st.reshape(-1)))
+        masks_changed = False
         for name, bit in (("m_fire", L.FIRE_OFF), ("m_out", L.OUT_OFF),
                           ("m_in", L.IN_OFF), ("m_hold", L.HOLD)):
             if name in drop or name not in state:
                 continue
-            m = torch.as_tensor(np.asarray(state[name]) == 0,
-                                device=self.device).to(torch.uint8)
-            f = s.view(s.fl)
-            f.copy_((f & (0xFF ^ bit)) | (m * bit))
+            off = (np.asarray(state[name]).reshape(B, n).T == 0)
+            f = s.fl_host[:, :B]
+            f &= np.uint8(0xFF ^ bit)
+            f |= off.astype(np.uint8) * np.uint8(bit)
+            masks_changed = True
             done.append(name)
+        if masks_changed:
+            s._push_flags()
         if "v_hold" not in drop and "v_hold" in state:
-            vh = np.asarray(state["v_hold"], np.float32)
-            if np.any(vh != 0) and s.vhold is None:
-                s.vhold = torch.zeros(s.nb, dtype=torch.float32,
-                                      device=self.device)
-                s._sync_struct()
-            if s.vhold is not None:
-                put(s.vhold, vh, torch.float32)
+            vh = np.asarray(state["v_hold"], np.float32).reshape(B, n)
+            s.vhold_host[:, :B] = vh.T
+            s.vhold.copy_(torch.from_numpy(s.vhold_host.reshape(-1)))
             done.append("v_hold")
         self._set_clock(T)
         if "delay_buffer" not in drop and "delay_buffer" in state:
             self._restore_ring(np.asarray(state["delay_buffer"], np.float32), T)
             done.append("delay_buffer")
         if s.spikes_out is not None:
-            s.spikes_out.view(B, n).copy_(
-                s.spike_state().to(torch.float32))
+            s.spikes_out.copy_(torch.from_numpy(
+                ((st & L.ST_SPIKE) != 0).astype(np.float32).reshape(-1)))
         if "calls" in state:
             self.calls = int(state["calls"])
         return done
@@ -551,7 +553,7 @@ class Engine:
         """
         s = self._sim
         w = np.float32(self.w_scale)
-        s.ring.zero_()
+        ring = np.zeros((L.RING_LEN, s.n, s.bstride), np.int32)
         for u in range(T - self.buf_len, T):
             if u < 0:
                 continue
@@ -562,8 +564,8 @@ class Engine:
                 raise ValueError("delay_buffer slot %d is not a multiple of "
                                  "wScale: not a state of this model"
                                  % (u % self.buf_len))
-            s.rec_slot(u).copy_(torch.as_tensor(I.astype(np.int32),
-                                                device=self.device))
+            ring[u % L.RING_LEN, :, :self.batch] = I.T.astype(np.int32)
+        s.ring.copy_(torch.from_numpy(ring.reshape(-1)))
         s.seed_ring(T % L.RING_LEN)
 
 

@@ -38,6 +38,16 @@ extern "C" {
 #define FIS_MAX_WINDOW 20   /* steps per window: delay horizon 19 + 1     */
 #define FIS_RING_LEN 20     /* int32 ring slots: rec(t-19) .. rec(t)       */
 
+/* Line rule.  Within one kernel, outputs written by different workgroups
+ * never share a 128-byte line (on MI355X such plain stores from different
+ * XCDs were measured to overwrite each other).  Hence the per-block layouts
+ * below: a workgroup is FIS_BLOCK_WAVES waves, and per-wave records of one
+ * block are packed into a line of their own. */
+#define FIS_BLOCK_WAVES 4   /* 256 threads per workgroup                  */
+#define FIS_LINE_WORDS 16   /* 128 bytes = 16 x 8-byte words              */
+#define FIS_LINE_INTS 32    /* 128 bytes = 32 x 4-byte ints               */
+#define FIS_EVENT_ALIGN 8   /* 8 events = 128 bytes                       */
+
 /* flag bits of `fl`, one byte per neuron-trial (sim.py:393-410, :866-900) */
 #define FIS_FIRE_OFF 0x01u  /* m_fire == 0: spike output suppressed        */
 #define FIS_OUT_OFF 0x02u   /* m_out  == 0: spike not transmitted          */
@@ -69,7 +79,9 @@ typedef struct {
   int32_t n;         /* neurons                                        */
   int32_t batch;     /* trials in use                                  */
   int32_t bstride;   /* trial stride (padded batch)                    */
-  int32_t ev_cap;    /* event capacity per update wave                 */
+  int32_t ev_cap;    /* event slots per update wave: a multiple of 8,
+                        >= 20 * max(1, 64 / bstride) + 1 (the last slot
+                        of each wave's segment is a sink)               */
   /* connectome by presynaptic neuron (W^T CSR) */
   const int64_t* offsets; /* [n+1]                                     */
   const int32_t* cols;    /* [E] postsynaptic neuron                   */
@@ -83,7 +95,8 @@ typedef struct {
   int32_t* ring;          /* [FIS_RING_LEN][n * bstride]               */
   uint32_t* sbits;        /* [n * bstride] kick bits of the window     */
   fis_event* events;      /* [n_waves * ev_cap]                        */
-  int32_t* ev_count;      /* [n_waves]                                 */
+  int32_t* ev_count;      /* [n_blocks][32]: count of wave w at
+                             [w / 4][w % 4]                            */
   int64_t* clock;         /* [1] first step of the next window         */
   uint32_t* done;         /* [1] completion counter, starts at 0       */
   /* per-trial Philox key and stream id, [bstride] */
@@ -101,12 +114,14 @@ typedef struct {
   float prob_scale;  /* DT/1000                      1e-4            */
   int32_t refrac_steps; /* round(tRefrac/DT)         22              */
   /* optional outputs (nullable) */
-  float* spikes_out;    /* [batch][n] float 0/1 of the window's last step */
+  float* spikes_out;    /* [n * bstride] float 0/1 of the window's last
+                           step, same layout as v                      */
   uint32_t* counts;     /* [n * bstride] spike count per neuron-trial  */
   uint32_t* pop_counts; /* [n] spike count per neuron, all trials      */
-  uint64_t* raster;     /* [raster_len][n_waves] ballot of every spike,
-                           row (t mod raster_len)                      */
-  int32_t raster_len;
+  uint64_t* raster;     /* [raster_len][n_blocks][16]: spikes of wave w
+                           at step t are bit j (lane j) of word
+                           [t mod raster_len][w / 4][w % 4]             */
+  int32_t raster_len;   /* >= FIS_MAX_WINDOW                           */
 } fis_sim;
 
 /* Drive table entry: neuron-trial `idx` is driven at rate
@@ -173,6 +188,8 @@ int64_t fis_struct_layout(int what);
 
 /* Number of update waves (event segments) for a given n and bstride. */
 int64_t fis_n_waves(int32_t n, int32_t bstride);
+/* Number of update workgroups (FIS_BLOCK_WAVES waves each). */
+int64_t fis_n_blocks(int32_t n, int32_t bstride);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,149 @@
+"""The drop-in engine surface on the GPU: Engine.step / use_graphs / masks /
+snapshot-restore / run, against the oracle and the reference's golden
+vectors, through the C ABI."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from flybrain_amd import connectome as C
+from flybrain_amd import sim as S
+from gpu_helpers import tiny_conn
+
+pytestmark = pytest.mark.gpu
+
+
+def test_engine_step_matches_oracle_philox():
+    """Engine.step(rates) with masks changing, B=3, against the oracle fed
+    the same Philox stream; the graph road gives the same bytes."""
+    conn = C.synthetic(400, None, mean_out=40, count_exp=1.2, n_groups=3,
+                       seed=8)
+    n, B = conn.n, 3
+    drv = np.arange(0, n, 9)
+    rates = torch.zeros(B, n, device="cuda")
+    rates[:, drv] = 600.0
+    rates[1, drv[::2]] = 2500.0
+    outs = []
+    for graphs in (False, True):
+        eng = S.Engine(conn, n, batch=B, seed=1234, no_refractory=drv)
+        if graphs:
+            assert eng.use_graphs()
+        ora = O.OracleEngine(conn, B, no_refractory=drv,
+                             seeds=[1234] * B)
+        for t in range(300):
+            if t == 100:
+                eng.set_masks(fire=[5, 6], out=[10, 11, 12], inc=[20],
+                              hold=[30, 31], v_clamp=-58.0)
+                ora.set_masks(fire=[5, 6], out=[10, 11, 12], inc=[20],
+                              hold=[30, 31], v_clamp=-58.0)
+            if t == 200:
+                eng.set_masks(fire=[], out=[], inc=[], hold=[])
+                ora.set_masks(fire=[], out=[], inc=[], hold=[])
+            sp = eng.step(rates)
+            ora.step_rates(rates.cpu().numpy())
+            assert np.array_equal(sp.cpu().numpy(), ora.spikes), "t=%d" % t
+        np.testing.assert_array_equal(eng.v.cpu().numpy().view(np.uint32),
+                                      ora.v.view(np.uint32))
+        np.testing.assert_array_equal(
+            eng.conductance.cpu().numpy().view(np.uint32),
+            ora.conductance.view(np.uint32))
+        db = eng.delay_buffer.cpu().numpy()
+        np.testing.assert_array_equal(db + np.float32(0), ora.delay_buffer
+                                      + np.float32(0))
+        assert eng.ptr == ora.ptr and eng.calls == 300
+        outs.append(eng.v.cpu().numpy().copy())
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_snapshot_restore_continues_bit_exact():
+    """Resume from a snapshot into a fresh engine: the continuation is the
+    same bytes (probe/resume_state.py); dropping the ring breaks it."""
+    conn = C.synthetic(500, None, mean_out=40, count_exp=1.2, n_groups=3,
+                       seed=6)
+    n, B = conn.n, 2
+    drv = np.arange(0, n, 11)
+    rates = torch.zeros(B, n, device="cuda")
+    rates[:, drv] = 900.0
+    a = S.Engine(conn, n, batch=B, seed=77, no_refractory=drv)
+    for _ in range(137):
+        a.step(rates)
+    snap = a.snapshot()
+    ref = []
+    for _ in range(120):
+        ref.append(a.step(rates).cpu().numpy().copy())
+    b = S.Engine(conn, n, batch=B, seed=5, no_refractory=drv)
+    done = b.restore(snap)
+    assert {"v", "conductance", "spikes", "refrac", "delay_buffer", "ptr",
+            "gen"} <= set(done)
+    for k in range(120):
+        assert np.array_equal(b.step(rates).cpu().numpy(), ref[k]), k
+    c = S.Engine(conn, n, batch=B, seed=5, no_refractory=drv)
+    c.restore(snap, drop=("delay_buffer",))
+    differs = any(not np.array_equal(c.step(rates).cpu().numpy(), ref[k])
+                  for k in range(120))
+    assert differs, "dropping the ring must change the continuation"
+
+
+def test_run_matches_reference_golden(golden):
+    """run() fed the reference's recorded Bernoulli tape: per-bin int16
+    raster and per-neuron counts equal the reference's run()."""
+    z = golden("run_sched.npz")
+    conn = C.Connectome(int(z["n"]), z["conn_offsets"], z["conn_cols"],
+                        z["conn_vals"])
+    n, B = conn.n, int(z["batch"])
+    st = [S.Stimulus(indices=z["stim_a"], rate_hz=400.0, t_on_ms=5.0,
+                     t_off_ms=32.5, name="a"),
+          S.Stimulus(indices=z["stim_b"], rate_hz=900.0, t_on_ms=20.0,
+                     t_off_ms=None, name="b")]
+    tape = z["tape_bits"]  # [steps, B, packed n]
+    bits = np.unpackbits(tape, axis=-1)[..., :n]
+    t, b, i = np.nonzero(bits)
+    cfg = S.RunConfig(t_run_s=float(z["t_run_s"]), bin_ms=float(z["bin_ms"]),
+                      n_trials=B, seed=0, device="cuda")
+    got_bins = []
+    res = S.run(st, silenced=z["silenced"], config=cfg, weights=conn,
+                tape=(t, i, b), on_bin=lambda k, ms, c: got_bins.append(k))
+    np.testing.assert_array_equal(res.counts, z["counts"])
+    np.testing.assert_array_equal(res.bins, z["bins"])
+    assert got_bins == list(range(z["bins"].shape[0]))
+
+
+def test_engine_refuses_cpu_and_bad_roads():
+    conn = C.synthetic(50, None, mean_out=5, n_groups=0, seed=1)
+    with pytest.raises(S.KernelUnavailable):
+        S.Engine(conn, 50, device="cpu")
+    with pytest.raises(ValueError):
+        S.Engine(conn, 50, spmv="nope")
+
+
+def test_tiny_masks_through_engine(golden):
+    """The Engine surface on the tiny fixture (step_tape, set_masks): v,
+    g and spikes fingerprints equal the reference's at every step."""
+    import hashlib
+
+    from flybrain_amd.device import Tape
+    from gpu_helpers import mask_schedule
+
+    z = golden("tiny_masks.npz")
+    conn = tiny_conn(z)
+    n, B = conn.n, int(z["batch"])
+    bits = np.unpackbits(z["tape_bits"], axis=-1)[..., :n]
+    eng = S.Engine(conn, n, batch=B, no_refractory=z["no_refractory"])
+    t_, b_, i_ = np.nonzero(bits)
+    tape = Tape(eng._sim, t_, i_, b_)
+    sched = mask_schedule(z)
+
+    def fp(a):
+        return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()[:16]
+    for t in range(int(z["steps"])):
+        if t in sched:
+            eng.set_masks(**sched[t])
+        sp = eng.step_tape(tape)
+        assert fp(eng.v.cpu().numpy()) == z["fps"][t][0], t
+        assert fp(eng.conductance.cpu().numpy()) == z["fps"][t][1], t
+        assert fp(sp.cpu().numpy()) == z["fps"][t][2], t
+        db = eng.delay_buffer.cpu().numpy() + np.float32(0)
+        assert fp(db) == z["fps"][t][4], t

@@ -144,8 +144,7 @@ def test_philox_drive_batched(batch):
     sim = DeviceSim(net, batch, seeds=seeds, raster_len=40)
     ne = np.concatenate(drv)
     tr = np.concatenate([np.full(len(d), b) for b, d in enumerate(drv)])
-    sim.set_flag_pairs(0x10, ne, tr)  # refractory exemption per trial
-    sim.st.view(n, sim.bstride)[torch.as_tensor(ne), torch.as_tensor(tr)] = 0
+    sim.exempt_pairs(ne, tr)  # refractory exemption per trial
     table = DriveTable(sim, ne, tr, tr, np.array(rate, np.float32))
     ora = O.OracleEngine(conn, batch, seeds=seeds, no_refractory_trials=drv)
     rates = np.zeros((batch, n), np.float32)
@@ -168,8 +167,8 @@ def test_rates_drive_matches_table():
     conn = C.synthetic(300, None, mean_out=30, n_groups=0, seed=3)
     n, B = conn.n, 2
     net = DeviceNet(conn)
-    a = DeviceSim(net, B, seeds=[11, 12], raster_len=8)
-    b = DeviceSim(net, B, seeds=[11, 12], raster_len=8)
+    a = DeviceSim(net, B, seeds=[11, 12], raster_len=20)
+    b = DeviceSim(net, B, seeds=[11, 12], raster_len=20)
     rates = torch.zeros(B, n, device="cuda")
     idx = np.arange(0, n, 7)
     rates[0, idx] = 2000.0

@@ -278,3 +278,21 @@ def test_rate_limit_is_loud():
         check_rates_hz(10001.0, "x")
     with pytest.raises(RateTooHigh):
         check_rates_hz([-1.0, 5.0], "x")
+
+
+def test_torch_restatement_matches_reference(golden):
+    """The torch-CPU restatement (the CPU baseline) reproduces the golden
+    fingerprints on the 10k workload for the first 300 steps."""
+    import ref_torch
+
+    z = golden("small10k.npz")
+    conn = C.small()
+    eng = ref_torch.TorchRef(conn, 1, no_refractory=z["driven"])
+    for t in range(300):
+        b = np.zeros((1, conn.n), np.uint8)
+        b[0, z["tape_i"][z["tape_t"] == t]] = 1
+        eng.step(b)
+        got = [fp(eng.v.numpy()), fp(eng.conductance.numpy()),
+               fp(eng.spikes.numpy()), fp(eng.refrac.numpy()),
+               fp(canon(eng.delay_buffer.numpy()))]
+        assert got == list(z["fps"][t]), "step %d" % t
