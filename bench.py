@@ -122,7 +122,53 @@ def cpu_baseline(conn, wl, seconds):
             "ms_per_step": round(1e3 * wall / steps, 3)}
 
 
+def summarize(paths):
+    """Per-kernel summary of rocprofv3 databases (kernel trace or --pmc).
+
+    Dispatches are grouped by kernel name and grid size (which tells the
+    B = 256 and B = 1 workloads apart).  For counter databases the mean
+    value per dispatch is printed next to what the same counter read for
+    the 256 MiB calibration copy (``--calib-copy``).
+    """
+    import sqlite3
+
+    for path in paths:
+        con = sqlite3.connect(path)
+        print("# %s" % path)
+        rows = con.execute(
+            "select name, grid_x, count(*), avg(duration), min(duration), "
+            "max(duration), sum(duration) from kernels group by name, grid_x "
+            "order by sum(duration) desc").fetchall()
+        if rows and rows[0][2]:
+            print("%-60s %10s %7s %12s %12s %12s" % (
+                "kernel", "grid", "calls", "avg_us", "min_us", "max_us"))
+            for name, gx, n, avg, mn, mx, tot in rows[:14]:
+                print("%-60s %10d %7d %12.3f %12.3f %12.3f" % (
+                    _short(name), gx, n, avg / 1e3, mn / 1e3, mx / 1e3))
+        pmc = con.execute(
+            "select kernel_name, grid_size, counter_name, count(*), avg(value) "
+            "from counters_collection group by kernel_name, grid_size, "
+            "counter_name order by kernel_name").fetchall()
+        if pmc:
+            print("%-60s %12s %-12s %7s %16s" % ("kernel", "grid", "counter",
+                                                 "calls", "mean_value"))
+            for name, gs, cn, n, val in pmc:
+                if n and ("fis" in name or "k_" in name
+                          or "elementwise" in name):
+                    print("%-60s %12d %-12s %7d %16.1f" % (
+                        _short(name), gs, cn, n, val))
+        print()
+
+
+def _short(name):
+    name = name.replace("(anonymous namespace)::", "")
+    return name[:60]
+
+
 def main():
+    if len(sys.argv) > 2 and sys.argv[1] == "--summarize":
+        summarize(sys.argv[2:])
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5000)
@@ -132,6 +178,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=("all", "b256", "b1"), default="all")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="eager launches (for per-dispatch counter runs)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a 256 MiB elementwise copy (counter "
+                         "calibration: known bytes read and written)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the whole batch on "
@@ -168,9 +219,11 @@ def main():
             wl = WL.plain_b1(conn.n, seed=0, steps=args.warmup + args.b1_steps)
             steps = args.b1_steps
         r = BatchRunner(net, wl, counts=True)
-        r.capture(args.graph_windows)
-        r.advance(args.warmup)
-        wall = timed(lambda: r.advance(steps), sync)
+        if not args.no_graph:
+            r.capture(args.graph_windows)
+        r.advance(args.warmup, use_graph=not args.no_graph)
+        wall = timed(lambda: r.advance(steps, use_graph=not args.no_graph),
+                     sync)
         spikes = int(r.counts().sum())
         kt = kernel_times(r, 20)
         res[name] = dict(wl=wl, runner=r, wall=wall, steps=steps,
@@ -183,6 +236,13 @@ def main():
                spikes / (wl.batch * (steps + args.warmup)),
                {k: round(v, 5) for k, v in kt.items()}))
 
+    if args.calib_copy:
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = a + 1.0  # reads 256 MiB, writes 256 MiB
+        sync()
+        log("calibration copy: 256 MiB read + 256 MiB written (%s)"
+            % float(b[0]))
+        del a, b
     head = res.get("b256") or res.get("b1")
     wl = head["wl"]
     n, B, W = conn.n, wl.batch, head["runner"].window

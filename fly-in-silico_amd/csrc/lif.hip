@@ -342,24 +342,16 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
   if (lane == 0 && act)
     s.ev_count[(int64_t)blockIdx.x * FIS_LINE_INTS + (wave & 3)] = nev;
 
-  // the last block to finish advances the clock for the next window
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const uint32_t prev = atomicAdd(s.done, 1u);
-    if (prev == gridDim.x - 1) {
-      *s.clock = t0 + W;
-      *s.done = 0u;
-      __threadfence();
-    }
-  }
 }
 
 // ------------------------------------------------------------- scatter
 
 // One wave per update-wave segment (grid-stride); each event's CSR row is
 // read once and added into every trial of its mask.
-__global__ __launch_bounds__(kBlock) void k_scatter(fis_sim s) {
+__global__ __launch_bounds__(kBlock) void k_scatter(fis_sim s, int W) {
+  // the window is complete: the next drive and update start W steps later
+  // (no kernel of this launch reads the clock)
+  if (blockIdx.x == 0 && threadIdx.x == 0) *s.clock += W;
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t nw = n_waves_of(s);
   const int64_t nb = (int64_t)s.n * s.bstride;  // ring slot stride
@@ -458,7 +450,7 @@ int check_sim(const fis_sim* s) {
     return (int)hipErrorInvalidValue;
   if (!s->offsets || !s->cols || !s->vals || !s->v || !s->g || !s->st ||
       !s->fl || !s->ring || !s->sbits || !s->events || !s->ev_count ||
-      !s->clock || !s->done || !s->seeds || !s->trial_ids)
+      !s->clock || !s->seeds || !s->trial_ids)
     return (int)hipErrorInvalidValue;
   if (s->raster && s->raster_len < FIS_MAX_WINDOW)
     return (int)hipErrorInvalidValue;
@@ -574,13 +566,14 @@ int fis_update(const fis_sim* s, int W, void* stream) {
   return (int)hipGetLastError();
 }
 
-int fis_scatter(const fis_sim* s, void* stream) {
+int fis_scatter(const fis_sim* s, int W, void* stream) {
   if (int rc = check_sim(s)) return rc;
+  if (W < 1 || W > FIS_MAX_WINDOW) return (int)hipErrorInvalidValue;
   const int64_t nw = fis_n_waves(s->n, s->bstride);
   int64_t blocks = (nw + (kBlock / kWave) - 1) / (kBlock / kWave);
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(k_scatter, dim3((unsigned)blocks), dim3(kBlock), 0,
-                     (hipStream_t)stream, *s);
+                     (hipStream_t)stream, *s, W);
   return (int)hipGetLastError();
 }
 

@@ -73,7 +73,7 @@ class FisSim(ctypes.Structure):
         ("offsets", P), ("cols", P), ("vals", P),
         ("v", P), ("g", P), ("st", P), ("fl", P), ("vhold", P),
         ("ring", P), ("sbits", P), ("events", P), ("ev_count", P),
-        ("clock", P), ("done", P), ("seeds", P), ("trial_ids", P),
+        ("clock", P), ("seeds", P), ("trial_ids", P),
         ("w_scale", ctypes.c_float), ("a_syn", ctypes.c_float),
         ("a_mem", ctypes.c_float), ("v_rest", ctypes.c_float),
         ("v_reset", ctypes.c_float), ("v_th", ctypes.c_float),
@@ -121,7 +121,7 @@ def lib():
         "fis_drive_table": ([sims, P, I, P, I, I, I, I, P], I),
         "fis_drive_tape": ([sims, P, P, I64, I, P], I),
         "fis_update": ([sims, I, P], I),
-        "fis_scatter": ([sims, P], I),
+        "fis_scatter": ([sims, I, P], I),
         "fis_seed_ring": ([sims, I, P], I),
     }
     for name, (args, res) in sig.items():

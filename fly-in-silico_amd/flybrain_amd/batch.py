@@ -64,7 +64,7 @@ class BatchRunner:
         back afterwards, as Engine.use_graphs does (sim.py:680-697).
         """
         s = self.sim
-        names = ["v", "g", "st", "ring", "sbits", "ev_count", "clock", "done"]
+        names = ["v", "g", "st", "ring", "sbits", "ev_count", "clock"]
         names += [k for k in ("counts", "pop_counts", "raster")
                   if getattr(s, k) is not None]
         saved = {k: getattr(s, k).clone() for k in names}

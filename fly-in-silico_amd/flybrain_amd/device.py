@@ -100,7 +100,6 @@ class DeviceSim:
         self.ev_count = torch.zeros(self.n_blocks * L.LINE_INTS,
                                     dtype=torch.int32, **z)
         self.clock = torch.zeros(1, dtype=torch.int64, **z)
-        self.done = torch.zeros(1, dtype=torch.int32, **z)
         sd = np.zeros(bs, np.uint64)
         sd[:B] = 0 if seeds is None else np.asarray(seeds, np.uint64)
         self.seeds_host = sd
@@ -122,6 +121,7 @@ class DeviceSim:
                                    * L.LINE_WORDS, dtype=torch.int64, **z)
                        if raster_len else None)
         self.t = 0  # host mirror of the device clock
+        self._open_window = 0
 
         # Host mirrors of the host-written per-neuron-trial arrays.  Changes
         # are made here and sent down as one contiguous copy: scattered
@@ -150,7 +150,7 @@ class DeviceSim:
         s.cols = self.net.cols.data_ptr()
         s.vals = self.net.vals.data_ptr()
         for name in ("v", "g", "st", "fl", "ring", "sbits", "events",
-                     "ev_count", "clock", "done", "seeds", "trial_ids"):
+                     "ev_count", "clock", "seeds", "trial_ids"):
             setattr(s, name, getattr(self, name).data_ptr())
         s.vhold = self.vhold.data_ptr() if self.vhold is not None else None
         for name in ("w_scale", "a_syn", "a_mem", "v_rest", "v_reset", "v_th",
@@ -258,11 +258,15 @@ class DeviceSim:
                                         _stream()), "fis_drive_tape")
 
     def update(self, W):
+        """W update steps; the window closes with :meth:`scatter`."""
         L.check(self.lib.fis_update(self._sp, W, _stream()), "fis_update")
-        self.t += W
+        self._open_window = W
 
     def scatter(self):
-        L.check(self.lib.fis_scatter(self._sp, _stream()), "fis_scatter")
+        """Scatter the window's spikes and advance the clock by W."""
+        W = self._open_window
+        L.check(self.lib.fis_scatter(self._sp, W, _stream()), "fis_scatter")
+        self.t += W
 
     def window(self, W, drive=None):
         """One window: drive (a DriveTable, Tape or rates tensor), update,

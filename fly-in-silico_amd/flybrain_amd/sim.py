@@ -405,7 +405,7 @@ class Engine:
     def _device_snapshot(self):
         s = self._sim
         names = ["v", "g", "st", "ring", "sbits", "ev_count", "clock",
-                 "done", "spikes_out"]
+                 "spikes_out"]
         return ({k: getattr(s, k).clone() for k in names}, s.t)
 
     def _device_restore(self, snap):

@@ -98,7 +98,6 @@ typedef struct {
   int32_t* ev_count;      /* [n_blocks][32]: count of wave w at
                              [w / 4][w % 4]                            */
   int64_t* clock;         /* [1] first step of the next window         */
-  uint32_t* done;         /* [1] completion counter, starts at 0       */
   /* per-trial Philox key and stream id, [bstride] */
   const uint64_t* seeds;
   const uint32_t* trial_ids;
@@ -170,12 +169,13 @@ int fis_drive_tape(const fis_sim* s, const int64_t* tape_ptr,
                    void* stream);
 
 /* W steps of the neuron update (Engine._core minus the SpMV,
- * sim.py:920-953), spike compaction into events, clock += W. */
+ * sim.py:920-953) starting at step *clock, spike compaction into events. */
 int fis_update(const fis_sim* s, int W, void* stream);
 
-/* Scatter of the last window's events into the int32 ring
- * (the SpMV of sim.py:906-918 / ev_spmv.cu:60-79, integer, exact). */
-int fis_scatter(const fis_sim* s, void* stream);
+/* Scatter of the events of the window just updated (W steps) into the
+ * int32 ring (the SpMV of sim.py:906-918 / ev_spmv.cu:60-79, integer,
+ * exact), then clock += W. */
+int fis_scatter(const fis_sim* s, int W, void* stream);
 
 /* Scatter the current spike state (st bit 7, not FIS_OUT_OFF) into ring
  * slot `slot` -- used after a restore to rebuild the input in flight. */

@@ -49,7 +49,7 @@ def test_bad_arguments_rejected_before_launch(lib):
     s = L.FisSim()
     s.n, s.batch, s.bstride = 10, 3, 3  # 3 is not a valid stride
     assert lib.fis_update(ctypes.byref(s), 1, None) != 0
-    assert lib.fis_scatter(ctypes.byref(s), None) != 0
+    assert lib.fis_scatter(ctypes.byref(s), 1, None) != 0
     assert lib.fis_ev_spmv_i32(None, None, None, None, None, None, 0, 1, 128,
                                None) != 0
 
