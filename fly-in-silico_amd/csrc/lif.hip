@@ -185,7 +185,7 @@ __device__ __forceinline__ int prefix_count(bool p, int lane, int& total) {
 // nothing to write), so no branch narrows the wave around them.
 __device__ __forceinline__ void emit(const fis_sim& s, bool fired, bool sends,
                                      int lane, int pre, int tb, int wave,
-                                     int nblk, int rrow, int slot,
+                                     int nblk, int rrow, int slot, int k,
                                      fis_event* evs, int& nev) {
   const int bs = s.bstride;
   if (s.raster) {
@@ -193,17 +193,21 @@ __device__ __forceinline__ void emit(const fis_sim& s, bool fired, bool sends,
     s.raster[((int64_t)rrow * nblk + (wave >> 2)) * FIS_LINE_WORDS +
              (wave & 3)] = all;
   }
-  if (!any_lane(sends)) return;
+  const bool any = any_lane(sends);
   if (bs >= kWave) {
+    // one wave = one neuron x 64 trials: at most one event per step, kept
+    // at position k of the wave's segment (empty mask when no spike), so
+    // the scatter can walk the window step by step
     fis_event e;
     e.pre = pre;
     e.slot = (uint16_t)slot;
     e.tb = (uint16_t)(tb - lane);  // trial of lane 0
-    e.mask = group_or(sends ? (1ull << lane) : 0ull, kWave);
-    evs[nev] = e;  // every lane writes the same record
-    nev += 1;
-  } else {
-    // 64/bs neurons per wave; the first lane of each neuron leads
+    e.mask = any ? group_or(sends ? (1ull << lane) : 0ull, kWave) : 0ull;
+    evs[k] = e;  // every lane writes the same record
+    return;
+  }
+  if (!any) return;
+  {    // 64/bs neurons per wave; the first lane of each neuron leads
     const uint64_t sub = group_or(sends ? (1ull << tb) : 0ull, bs);
     const bool lead = tb == 0 && sub != 0;
     int total;
@@ -324,7 +328,7 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
     const bool fired = act && spk != 0.0f;
     nspk += fired ? 1u : 0u;
     emit(s, fired, fired && transmit, lane, pre, tb, wave, nblk, rrow, slot,
-         evs, nev);
+         k, evs, nev);
     rrow = (rrow + 1 == s.raster_len) ? 0 : rrow + 1;
     slot = (slot + 1 == FIS_RING_LEN) ? 0 : slot + 1;
   }
@@ -339,23 +343,42 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
       if (s.pop_counts) atomicAdd(&s.pop_counts[pre], nspk);
     }
   }
-  if (lane == 0 && act)
+  if (bs < kWave && lane == 0 && act)
     s.ev_count[(int64_t)blockIdx.x * FIS_LINE_INTS + (wave & 3)] = nev;
 
 }
 
 // ------------------------------------------------------------- scatter
 
-// One wave per update-wave segment (grid-stride); each event's CSR row is
-// read once and added into every trial of its mask.
+// One event's CSR row, read once and added into every trial of its mask.
+__device__ __forceinline__ void scatter_event(const fis_sim& s,
+                                              const fis_event& ev, int lane) {
+  const int64_t nb = (int64_t)s.n * s.bstride;  // ring slot stride
+  const int bs = s.bstride;
+  const int64_t beg = s.offsets[ev.pre], end = s.offsets[ev.pre + 1];
+  int32_t* base = s.ring + (int64_t)ev.slot * nb + ev.tb;
+  for (int64_t k = beg + lane; k < end; k += kWave) {
+    const int32_t post = s.cols[k];
+    const int32_t val = s.vals[k];
+    int32_t* p = base + (int64_t)post * bs;
+    uint64_t m = ev.mask;
+    while (m) {
+      const int j = __ffsll((long long)m) - 1;
+      m &= m - 1ull;
+      __hip_atomic_fetch_add(p + j, val, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// bstride < 64: one wave per update-wave segment (grid-stride) over the
+// compacted events of the window.
 __global__ __launch_bounds__(kBlock) void k_scatter(fis_sim s, int W) {
   // the window is complete: the next drive and update start W steps later
   // (no kernel of this launch reads the clock)
   if (blockIdx.x == 0 && threadIdx.x == 0) *s.clock += W;
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t nw = n_waves_of(s);
-  const int64_t nb = (int64_t)s.n * s.bstride;  // ring slot stride
-  const int bs = s.bstride;
   const int64_t stride = (int64_t)gridDim.x * (kBlock / kWave);
   for (int64_t w = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6; w < nw;
        w += stride) {
@@ -363,23 +386,25 @@ __global__ __launch_bounds__(kBlock) void k_scatter(fis_sim s, int W) {
         s.ev_count[(w >> 2) * FIS_LINE_INTS + (w & 3)]);
     if (cnt == 0) continue;
     const fis_event* evs = s.events + w * s.ev_cap;
-    for (int e = 0; e < cnt; ++e) {
-      const fis_event ev = evs[e];
-      const int64_t beg = s.offsets[ev.pre], end = s.offsets[ev.pre + 1];
-      int32_t* base = s.ring + (int64_t)ev.slot * nb + ev.tb;
-      for (int64_t k = beg + lane; k < end; k += kWave) {
-        const int32_t post = s.cols[k];
-        const int32_t val = s.vals[k];
-        int32_t* p = base + (int64_t)post * bs;
-        uint64_t m = ev.mask;
-        while (m) {
-          const int j = __ffsll((long long)m) - 1;
-          m &= m - 1ull;
-          __hip_atomic_fetch_add(p + j, val, __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_AGENT);
-        }
-      }
-    }
+    for (int e = 0; e < cnt; ++e) scatter_event(s, evs[e], lane);
+  }
+}
+
+// bstride >= 64: step-major.  blockIdx.y is the window step; workgroups are
+// dispatched x-fastest, so at any time the atomics of the whole device fall
+// into one or two ring slots (142 MB each at 138,639 x 256), a working set
+// the 256 MB memory-side cache can hold.
+__global__ __launch_bounds__(kBlock) void k_scatter_steps(fis_sim s, int W) {
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *s.clock += W;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t nw = n_waves_of(s);
+  const int k = blockIdx.y;
+  const int64_t stride = (int64_t)gridDim.x * (kBlock / kWave);
+  for (int64_t w = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6; w < nw;
+       w += stride) {
+    const fis_event ev = s.events[w * s.ev_cap + k];
+    if (ev.mask == 0) continue;
+    scatter_event(s, ev, lane);
   }
 }
 
@@ -571,9 +596,15 @@ int fis_scatter(const fis_sim* s, int W, void* stream) {
   if (W < 1 || W > FIS_MAX_WINDOW) return (int)hipErrorInvalidValue;
   const int64_t nw = fis_n_waves(s->n, s->bstride);
   int64_t blocks = (nw + (kBlock / kWave) - 1) / (kBlock / kWave);
-  if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(k_scatter, dim3((unsigned)blocks), dim3(kBlock), 0,
-                     (hipStream_t)stream, *s, W);
+  if (s->bstride >= kWave) {
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_scatter_steps, dim3((unsigned)blocks, (unsigned)W),
+                       dim3(kBlock), 0, (hipStream_t)stream, *s, W);
+  } else {
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_scatter, dim3((unsigned)blocks), dim3(kBlock), 0,
+                       (hipStream_t)stream, *s, W);
+  }
   return (int)hipGetLastError();
 }
 
