@@ -49,26 +49,31 @@ def timed(fn, sync):
 
 
 def kernel_times(runner, windows):
-    """Average durations (ms) of drive, update and scatter launches over
-    ``windows`` eager windows, bracketed by events on the launch stream."""
+    """Average durations (ms) of the drive launch and of the road's
+    launches -- push: update, scatter; pull: the fused window kernel --
+    over ``windows`` eager windows, bracketed by HIP events on the launch
+    stream."""
     import torch
 
     s = runner.sim
-    ev = {k: [] for k in ("drive", "update", "scatter")}
+    W = runner.window
+    stages = [("drive", lambda: runner.table is not None
+               and s.drive_table(runner.table, W))]
+    if s.road == "pull":
+        stages.append(("pull", lambda: s.pull(W)))
+    else:
+        stages += [("update", lambda: s.update(W)),
+                   ("scatter", lambda: s.scatter())]
+    ev = {k: [] for k, _ in stages}
     stream = torch.cuda.current_stream()
     for _ in range(windows):
-        marks = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        marks = [torch.cuda.Event(enable_timing=True)
+                 for _ in range(len(stages) + 1)]
         marks[0].record(stream)
-        if runner.table is not None:
-            s.drive_table(runner.table, runner.window)
-        marks[1].record(stream)
-        s.update(runner.window)
-        marks[2].record(stream)
-        s.scatter()
-        marks[3].record(stream)
-        ev["drive"].append((marks[0], marks[1]))
-        ev["update"].append((marks[1], marks[2]))
-        ev["scatter"].append((marks[2], marks[3]))
+        for i, (k, fn) in enumerate(stages):
+            fn()
+            marks[i + 1].record(stream)
+            ev[k].append((marks[i], marks[i + 1]))
     torch.cuda.synchronize()
     return {k: sum(a.elapsed_time(b) for a, b in v) / len(v)
             for k, v in ev.items()}
@@ -88,6 +93,31 @@ def scatter_bytes(conn, raster_rows):
         rows += int((8 + 6 * deg[any_t]).sum())
         edges_trials += int((deg[None, :] * sp).sum())
     return rows + 8 * edges_trials
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of ``kernel`` at its largest grid, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summaries committed in profiles/
+    (KiB per dispatch, scaled by what the same counters read for a plain
+    256 MiB copy: FETCH x 2.0, WRITE x 1.0).  None when not profiled."""
+    import glob
+
+    best = {}
+    for path in sorted(glob.glob(str(ROOT / "profiles" / "*_hbm_counters.md"))):
+        for line in open(path):
+            parts = line.split()
+            if len(parts) < 5 or parts[-3] not in ("FETCH_SIZE", "WRITE_SIZE"):
+                continue
+            head = line[:60]
+            if kernel + "(" not in head and kernel + "<" not in head:
+                continue
+            grid, name, val = int(parts[-4]), parts[-3], float(parts[-1])
+            if grid >= best.get(name, (0, 0))[0]:
+                best[name] = (grid, val)
+    if "FETCH_SIZE" not in best or "WRITE_SIZE" not in best:
+        return None
+    return int(1024 * (2.0 * best["FETCH_SIZE"][1]
+                       + 1.0 * best["WRITE_SIZE"][1]))
 
 
 def cpu_baseline(conn, wl, seconds):
@@ -178,6 +208,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=("all", "b256", "b1"), default="all")
+    ap.add_argument("--road", choices=("auto", "push", "pull"), default="auto",
+                    help="auto: pull road for 64..1024 trials, push below")
     ap.add_argument("--no-graph", action="store_true",
                     help="eager launches (for per-dispatch counter runs)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -205,7 +237,8 @@ def main():
     log("connectome %d neurons, %d edges, %s (%.1f s)"
         % (conn.n, conn.n_edges, conn.fingerprint(), time.perf_counter() - t0))
     net = DeviceNet(conn, "cuda")
-    out = {"metric": "sim trial-s per wall-s at 138k neurons, B=256 vs B=1",
+    out = {"metric": "sim trial-s per wall-s at 138k neurons, B=256 vs B=1; "
+                     "p50/p99 req latency",
            "unit": "trial-s/s", "n_gpus": 1, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32+i32",
            "data": "synthetic"}
@@ -218,7 +251,7 @@ def main():
         else:
             wl = WL.plain_b1(conn.n, seed=0, steps=args.warmup + args.b1_steps)
             steps = args.b1_steps
-        r = BatchRunner(net, wl, counts=True)
+        r = BatchRunner(net, wl, counts=True, road=args.road)
         if not args.no_graph:
             r.capture(args.graph_windows)
         r.advance(args.warmup, use_graph=not args.no_graph)
@@ -247,33 +280,35 @@ def main():
     wl = head["wl"]
     n, B, W = conn.n, wl.batch, head["runner"].window
     kt = head["kt"]
-    dom = max(("update", "scatter"), key=lambda k: kt[k])
-    if dom == "update":
-        alg = BETA * n * B * W
-    else:
-        r = head["runner"]
+    dom = max((k for k in kt if k != "drive"), key=lambda k: kt[k])
+    alg = 0
+    if dom in ("update", "pull"):
+        alg += BETA * n * B * W
+    if dom in ("scatter", "pull"):
         # spikes of the last window, from a short raster run
         from flybrain_amd.batch import BatchRunner as _BR
         rr = _BR(net, wl, counts=False, raster_len=W)
         rr.advance(min(wl.steps, args.warmup + head["steps"]) // W * W - W)
         rr.advance(W, use_graph=False)
         rows = rr.sim.read_raster(rr.t - W, rr.t)
-        alg = scatter_bytes(conn, rows)
+        alg += scatter_bytes(conn, rows)
         del rr
     achieved = alg / (kt[dom] * 1e-3) / 1e9
+    kname = {"update": "k_update", "pull": "k_pull"}.get(dom) or (
+        "k_scatter_steps" if head["runner"].sim.bstride >= 64 else "k_scatter")
     out.update({
         "value": round(head["tps"], 3),
         "steps": head["steps"], "warmup": args.warmup,
         "ms_per_step": round(1e3 * head["wall"] / head["steps"], 6),
         "config": {"workload": wl.name, "neurons": n,
                    "edges": conn.n_edges, "batch": B,
+                   "road": head["runner"].sim.road,
                    "window_steps": W, "graph_windows": args.graph_windows,
                    "connectome": conn.fingerprint()},
-        "roofline": {"bound": "hbm", "kernel": "k_update" if dom == "update"
-                     else "k_scatter", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": profiled_traffic(kname),
                      "algorithmic_bytes_per_launch": int(alg),
                      "avg_launch_ms": round(kt[dom], 5)},
         "kernels_ms": {k: round(v, 5) for k, v in kt.items()},

@@ -82,6 +82,16 @@ __device__ __forceinline__ uint32_t window_bits(int64_t t0, int W,
   return bits;
 }
 
+// trial b's own step count at global step t (0 at its slot's start)
+__device__ __forceinline__ int64_t origin_of(const fis_sim& s, int b) {
+  return s.t_origin ? s.t_origin[b] : 0;
+}
+
+// first global step at which trial b's spikes are no longer counted
+__device__ __forceinline__ int64_t end_of(const fis_sim& s, int b) {
+  return s.t_end ? s.t_end[b] : INT64_MAX;
+}
+
 __device__ __forceinline__ int n_waves_of(const fis_sim& s) {
   return (s.n * s.bstride + kWave - 1) >> 6;
 }
@@ -100,7 +110,8 @@ __global__ __launch_bounds__(kBlock) void k_drive_rates(fis_sim s,
   if (b < s.batch) {
     const float p = rates[(int64_t)b * s.n + i] * s.prob_scale;
     if (p > 0.0f)
-      bits = window_bits(*s.clock, W, (uint32_t)i, s.trial_ids[b], s.seeds[b],
+      bits = window_bits(*s.clock - origin_of(s, b), W, (uint32_t)i,
+                         s.trial_ids[b], s.seeds[b],
                          [p](int64_t) { return p; });
   }
   s.sbits[idx] = bits;
@@ -116,7 +127,8 @@ __global__ __launch_bounds__(kBlock) void k_drive_table(
   const int b = (int)e.idx - i * s.bstride;
   const float ps = s.prob_scale;
   const uint32_t bits = window_bits(
-      *s.clock, W, (uint32_t)i, s.trial_ids[b], s.seeds[b], [&](int64_t t) {
+      *s.clock - origin_of(s, b), W, (uint32_t)i, s.trial_ids[b], s.seeds[b],
+      [&](int64_t t) {
         int64_t seg = t / seg_steps;
         if (seg >= n_seg) seg = n_seg - 1;
         return rate_hz[seg * n_groups + e.group] * ps;
@@ -224,6 +236,94 @@ __device__ __forceinline__ void emit(const fis_sim& s, bool fired, bool sends,
   }
 }
 
+// Registers of one neuron-trial over a window.  The masks are the
+// reference's float factors (exactly 0 or 1, sim.py:393-410).
+struct Cell {
+  float v, g, sp;
+  int r, R;
+  float m_fire, m_in, m_hold, v_hold;
+};
+
+__device__ __forceinline__ void load_cell(const fis_sim& s, int idx, float& v,
+                                          float& g, uint32_t& st,
+                                          uint32_t& fl, uint32_t& sb,
+                                          float& vh) {
+  v = s.v[idx];
+  g = s.g[idx];
+  st = s.st[idx];
+  fl = s.fl[idx];
+  sb = s.sbits[idx];
+  if ((fl & FIS_HOLD) && s.vhold) vh = s.vhold[idx];
+  if (sb) s.sbits[idx] = 0u;  // the window's kicks are consumed
+}
+
+__device__ __forceinline__ Cell make_cell(const fis_sim& s, uint32_t fl,
+                                          float vh, uint32_t st, float v,
+                                          float g) {
+  Cell c;
+  c.v = v;
+  c.g = g;
+  c.m_fire = (fl & FIS_FIRE_OFF) ? 0.0f : 1.0f;
+  c.m_in = (fl & FIS_IN_OFF) ? 0.0f : 1.0f;
+  c.m_hold = (fl & FIS_HOLD) ? 0.0f : 1.0f;
+  c.v_hold = (fl & FIS_HOLD) ? vh : 0.0f;
+  c.R = (fl & FIS_NOREFRAC) ? 0 : s.refrac_steps;
+  c.r = (int)(st & FIS_ST_RMAX);
+  c.sp = (st & FIS_ST_SPIKE) ? 1.0f : 0.0f;
+  return c;
+}
+
+// One step of Engine._core for one neuron-trial (sim.py:920-953), given the
+// exact integer synaptic input I read from the delay line and the Poisson
+// kick of the step.  Every line is one float32 rounding in the reference's
+// order.  Returns the masked spike (0 or 1).
+__device__ __forceinline__ float lif_step(const fis_sim& s, Cell& c, int32_t I,
+                                          uint32_t kick) {
+  // refractory counter: r = (r + 1) * (1 - s), saturating
+  c.r = (c.sp != 0.0f) ? 0 : (c.r < FIS_ST_RMAX ? c.r + 1 : FIS_ST_RMAX);
+  const float gate = (c.r >= c.R) ? 1.0f : 0.0f;
+
+  // conductance with the delayed input (sim.py:927-929)
+  const float rec = s.w_scale * (float)I;
+  const float del = rec * c.m_in;
+  const float gd = c.g * s.a_syn;
+  const float dg = del * gate;
+  const float g_new = gd + dg;
+
+  // stim = w_scale * (bern * gain), sim.py:859-861
+  const float stim = s.w_scale * ((kick ? 1.0f : 0.0f) * s.gain);
+  // membrane with the OLD conductance (sim.py:937-939)
+  float vv = c.v + stim;
+  const float t1 = vv - s.v_rest;
+  const float t2 = c.g - t1;
+  const float t3 = s.a_mem * t2;
+  vv = vv + t3;
+  vv = vv * c.m_hold;  // hold before threshold (sim.py:944)
+  vv = vv + c.v_hold;
+  float spk = (vv > s.v_th) ? 1.0f : 0.0f;
+  spk = spk * c.m_fire;
+  const float t4 = vv - s.v_reset;  // reset by the masked spike
+  const float t5 = t4 * spk;
+  vv = vv - t5;
+  const float gs = g_new * spk;
+  c.g = g_new - gs;
+  c.v = vv;
+  c.sp = spk;
+  return spk;
+}
+
+__device__ __forceinline__ void store_cell(const fis_sim& s, const Cell& c,
+                                           int idx, int pre, uint32_t nspk) {
+  s.v[idx] = c.v;
+  s.g[idx] = c.g;
+  s.st[idx] = (uint8_t)((c.sp != 0.0f ? FIS_ST_SPIKE : 0u) | (uint32_t)c.r);
+  if (s.spikes_out) s.spikes_out[idx] = c.sp;
+  if (nspk) {
+    if (s.counts) s.counts[idx] += nspk;
+    if (s.pop_counts) atomicAdd(&s.pop_counts[pre], nspk);
+  }
+}
+
 // W steps per neuron-trial.  WMAX is the unrolled bound (1 or 20).
 // Index arithmetic is 32-bit: n * bstride < 2^31 is checked on the host.
 template <int WMAX>
@@ -241,29 +341,11 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
 
   float v = 0.0f, g = 0.0f, vh = 0.0f;
   uint32_t st = 0, fl = FIS_FIRE_OFF, sb = 0;
-  if (act) {
-    v = s.v[idx];
-    g = s.g[idx];
-    st = s.st[idx];
-    fl = s.fl[idx];
-    sb = s.sbits[idx];
-    if ((fl & FIS_HOLD) && s.vhold) vh = s.vhold[idx];
-  }
-  if (act && sb) s.sbits[idx] = 0u;
+  if (act) load_cell(s, idx, v, g, st, fl, sb, vh);
 
-  // masks as the reference's float factors (exactly 0 or 1)
-  const float m_fire = (fl & FIS_FIRE_OFF) ? 0.0f : 1.0f;
-  const float m_in = (fl & FIS_IN_OFF) ? 0.0f : 1.0f;
-  const float m_hold = (fl & FIS_HOLD) ? 0.0f : 1.0f;
-  const float v_hold = (fl & FIS_HOLD) ? vh : 0.0f;
-  const int R = (fl & FIS_NOREFRAC) ? 0 : s.refrac_steps;
+  Cell c = make_cell(s, fl, vh, st, v, g);
   const bool transmit = act && !(fl & FIS_OUT_OFF);
-  // stim = w_scale * (bern * gain), sim.py:859-861
-  const float kick1 = s.w_scale * (1.0f * s.gain);
-  const float kick0 = s.w_scale * (0.0f * s.gain);
-
-  int r = (int)(st & FIS_ST_RMAX);
-  float sp = (st & FIS_ST_SPIKE) ? 1.0f : 0.0f;
+  const int64_t t_stop = act ? end_of(s, tb) : 0;
 
   // ring reads run PREFETCH steps ahead of the recurrence (they do not
   // depend on it); the step loop itself stays rolled, which keeps the
@@ -295,54 +377,16 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
     }
     if (act && I != 0) rp[(int64_t)slot * nb] = 0;
 
-    // refractory counter: r = (r + 1) * (1 - s), saturating
-    r = (sp != 0.0f) ? 0 : (r < FIS_ST_RMAX ? r + 1 : FIS_ST_RMAX);
-    const float gate = (r >= R) ? 1.0f : 0.0f;
-
-    // conductance with the delayed input (sim.py:927-929)
-    const float rec = s.w_scale * (float)I;
-    const float del = rec * m_in;
-    const float gd = g * s.a_syn;
-    const float dg = del * gate;
-    const float g_new = gd + dg;
-
-    // membrane with the OLD conductance (sim.py:937-939)
-    const float stim = ((sb >> k) & 1u) ? kick1 : kick0;
-    float vv = v + stim;
-    const float t1 = vv - s.v_rest;
-    const float t2 = g - t1;
-    const float t3 = s.a_mem * t2;
-    vv = vv + t3;
-    vv = vv * m_hold;  // hold before threshold (sim.py:944)
-    vv = vv + v_hold;
-    float spk = (vv > s.v_th) ? 1.0f : 0.0f;
-    spk = spk * m_fire;
-    const float t4 = vv - s.v_reset;  // reset by the masked spike
-    const float t5 = t4 * spk;
-    vv = vv - t5;
-    const float gs = g_new * spk;
-    g = g_new - gs;
-    v = vv;
-    sp = spk;
-
+    const float spk = lif_step(s, c, I, (sb >> k) & 1u);
     const bool fired = act && spk != 0.0f;
-    nspk += fired ? 1u : 0u;
+    nspk += (fired && t0 + k < t_stop) ? 1u : 0u;
     emit(s, fired, fired && transmit, lane, pre, tb, wave, nblk, rrow, slot,
          k, evs, nev);
     rrow = (rrow + 1 == s.raster_len) ? 0 : rrow + 1;
     slot = (slot + 1 == FIS_RING_LEN) ? 0 : slot + 1;
   }
 
-  if (act) {
-    s.v[idx] = v;
-    s.g[idx] = g;
-    s.st[idx] = (uint8_t)((sp != 0.0f ? FIS_ST_SPIKE : 0u) | (uint32_t)r);
-    if (s.spikes_out) s.spikes_out[idx] = sp;
-    if (nspk) {
-      if (s.counts) s.counts[idx] += nspk;
-      if (s.pop_counts) atomicAdd(&s.pop_counts[pre], nspk);
-    }
-  }
+  if (act) store_cell(s, c, idx, pre, nspk);
   if (bs < kWave && lane == 0 && act)
     s.ev_count[(int64_t)blockIdx.x * FIS_LINE_INTS + (wave & 3)] = nev;
 
@@ -431,6 +475,313 @@ __global__ __launch_bounds__(kBlock) void k_seed_ring(fis_sim s, int slot) {
   }
 }
 
+// One wave per listed neuron-trial whose transmit flag changed after its
+// last spike: add (flag now clear) or subtract (flag now set) its row.
+__global__ __launch_bounds__(kBlock) void k_resend(fis_sim s,
+                                                   const int32_t* idx,
+                                                   int n_idx, int slot) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
+  if (w >= n_idx) return;
+  const int src = idx[w];
+  if (src < 0 || src >= s.n * s.bstride) return;  // not a neuron-trial
+  if (!(s.st[src] & FIS_ST_SPIKE)) return;
+  const int32_t sign = (s.fl[src] & FIS_OUT_OFF) ? -1 : 1;
+  const int pre = src / s.bstride;
+  const int b = src - pre * s.bstride;
+  const int64_t nb = (int64_t)s.n * s.bstride;
+  int32_t* base = s.ring + (int64_t)slot * nb + b;
+  for (int64_t k = s.offsets[pre] + lane; k < s.offsets[pre + 1]; k += kWave)
+    __hip_atomic_fetch_add(base + (int64_t)s.cols[k] * s.bstride,
+                           sign * (int32_t)s.vals[k], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ------------------------------------------------------------- pull road
+//
+// One launch per window.  Workgroup = FIS_PULL_TILE neuron-trials = P
+// neurons x all bstride trials (bstride a power of two, 64..1024), so the
+// tile is a contiguous idx range.  Records are laid out [pre][slot][word]:
+// one neuron's 40 steps x bstride bits (320 B at 64 trials) belong to the
+// workgroup that owns the neuron, and the 20 steps of one delay horizon
+// sit in a few lines for the gather.
+//
+//  1. gather: the tile's in-edges (W CSR by post) are walked once; for each
+//     in-edge whose presynaptic neuron was active at some step t0-20+j of
+//     the window's delay horizon (act word), the 64-trial record words of
+//     that step are read and the synapse count is added into LDS
+//     acc[j][post][trial] for every set trial bit -- exact int32, order-free;
+//  2. update: W steps of lif_step per neuron-trial with I = acc[j][...],
+//     state in registers, spike records of steps t0..t0+W-1 written;
+//  3. the tile's act words get the window's bits (device atomics: the act
+//     words of several tiles share lines).
+// Records and act bits of steps t0-20.. (read) and t0.. (written) are
+// disjoint slots mod 40 for W <= 20, so no workgroup reads what another
+// writes in the same launch.
+
+constexpr int kPull = FIS_PULL_TILE;
+constexpr int kRec = FIS_REC_SLOTS;
+constexpr uint64_t kMask40 = (1ull << kRec) - 1;
+
+// rotate the low 40 bits of x left by r (0 <= r < 40)
+__device__ __forceinline__ uint64_t rot40(uint64_t x, int r) {
+  x &= kMask40;
+  return r == 0 ? x : (((x << r) | (x >> (kRec - r))) & kMask40);
+}
+
+template <int WMAX>
+__global__ __launch_bounds__(kPull) void k_pull(fis_sim s, int W,
+                                                int ring_steps) {
+  // exactly WMAX x 4 KB of LDS and nothing else, so that two workgroups
+  // share a CU's 160 KB
+  __shared__ int32_t acc[WMAX * kPull];
+
+  const int bs = s.bstride;
+  const int P = kPull / bs;     // neurons per tile
+  const int words = bs >> 6;    // 64-trial words per neuron
+  const int tid = (int)threadIdx.x;
+  const int p0 = (int)blockIdx.x * P;
+  const int np = min(P, s.n - p0);
+  const int64_t t0 = *s.clock;
+  const int rbase = (int)((t0 + kRec - 20) % kRec);  // slot of step t0-20
+  const int wbase = (int)(t0 % kRec);                // slot of step t0
+  const int pstride = kRec * words;                  // words per neuron
+  const uint64_t wmask = (1ull << W) - 1;
+
+  for (int j = 0; j < W; ++j) acc[j * kPull + tid] = 0;
+  // the tile's in-edge offsets: uniform, held in scalar registers
+  int off[kPull / kWave + 1];
+#pragma unroll
+  for (int q = 0; q <= kPull / kWave; ++q)
+    off[q] = q <= np ? s.in_offsets[p0 + q] : 0x7fffffff;
+  __syncthreads();
+
+  {  // 1. gather; record loads of up to 4 steps are in flight per lane
+    const int w = tid & (words - 1);
+    const int G = kPull / words;
+    const int e1 = s.in_offsets[p0 + np];
+    for (int e = off[0] + tid / words; e < e1; e += G) {
+      const int pre = s.in_cols[e];
+      const int32_t val = s.in_vals[e];
+      int pl = 0;
+#pragma unroll
+      for (int q = 1; q < kPull / kWave; ++q) pl += e >= off[q] ? 1 : 0;
+      uint64_t m = rot40(s.act[pre], (kRec - rbase) % kRec) & wmask;
+      int32_t* ap = acc + pl * bs + (w << 6);
+      const uint64_t* rp = s.rec + (int64_t)pre * pstride + w;
+      while (m) {
+        int j[4];
+        uint64_t x[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          j[q] = m ? __ffsll((long long)m) - 1 : -1;
+          m &= m - 1ull;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          int sl = rbase + j[q];
+          sl = sl >= kRec ? sl - kRec : sl;
+          x[q] = j[q] >= 0 ? rp[sl * words] : 0ull;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          uint64_t bits = x[q];
+          while (bits) {
+            const int b = __ffsll((long long)bits) - 1;
+            bits &= bits - 1ull;
+            atomicAdd(ap + j[q] * kPull + b, val);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // 2. update
+  const int nb = s.n * bs;
+  const int idx = (int)blockIdx.x * kPull + tid;
+  const int lane = tid & (kWave - 1);
+  const int wave = idx >> 6;
+  const bool act = idx < nb;
+  const int pre = idx / bs;
+  const int tb = idx - pre * bs;
+  const int nblk = (nb + kBlock - 1) / kBlock;  // raster rows: 256-lane groups
+
+  float v = 0.0f, g = 0.0f, vh = 0.0f;
+  uint32_t st = 0, fl = FIS_FIRE_OFF, sb = 0;
+  if (act) load_cell(s, idx, v, g, st, fl, sb, vh);
+  Cell c = make_cell(s, fl, vh, st, v, g);
+  const bool transmit = act && !(fl & FIS_OUT_OFF);
+  const int64_t t_stop = act ? end_of(s, tb) : 0;
+
+  uint64_t* wp = s.rec + (int64_t)pre * pstride + (tb >> 6);
+  int32_t* rp = s.ring + idx;
+  int rslot = (int)((t0 + 1) % FIS_RING_LEN);
+  int wslot = wbase;
+  uint32_t nspk = 0;
+  uint32_t abits = 0;  // wave-uniform: steps at which this wave sent
+  int rrow = s.raster ? (int)(t0 % s.raster_len) : 0;
+
+#pragma unroll 1
+  for (int k = 0; k < W; ++k) {
+    int32_t I = acc[k * kPull + tid];
+    if (k < ring_steps && act) {  // input carried over from a restore
+      const int32_t x = rp[(int64_t)rslot * nb];
+      if (x != 0) {
+        I += x;
+        rp[(int64_t)rslot * nb] = 0;
+      }
+    }
+    const float spk = lif_step(s, c, I, (sb >> k) & 1u);
+    const bool fired = act && spk != 0.0f;
+    nspk += (fired && t0 + k < t_stop) ? 1u : 0u;
+    // `act` is uniform over a wave (n * bstride is a multiple of 64): the
+    // waves past the last neuron of the last tile have no raster word
+    if (s.raster && act) {
+      const uint64_t all = group_or(fired ? (1ull << lane) : 0ull, kWave);
+      s.raster[((int64_t)rrow * nblk + (wave >> 2)) * FIS_LINE_WORDS +
+               (wave & 3)] = all;
+    }
+    if (s.raster) rrow = (rrow + 1 == s.raster_len) ? 0 : rrow + 1;
+    const bool sends = fired && transmit;
+    const uint64_t msk = group_or(sends ? (1ull << lane) : 0ull, kWave);
+    if (act) wp[wslot * words] = msk;  // every lane: the same word
+    abits |= msk != 0 ? (1u << k) : 0u;
+    rslot = (rslot + 1 == FIS_RING_LEN) ? 0 : rslot + 1;
+    wslot = (wslot + 1 == kRec) ? 0 : wslot + 1;
+  }
+  if (act) store_cell(s, c, idx, pre, nspk);
+  // a thread's own step-0 cell is free once read: the first lane of each
+  // wave parks its step bits there
+  if (lane == 0) acc[tid] = (int32_t)abits;
+  __syncthreads();
+
+  // 3. act bits of the window's steps: clear, then set
+  if (tid < np) {
+    uint32_t bits32 = 0;
+    for (int q = 0; q < words; ++q) bits32 |= (uint32_t)acc[tid * bs + q * kWave];
+    uint64_t* a = s.act + p0 + tid;
+    __hip_atomic_fetch_and(a, ~rot40(wmask, wbase), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t bits = rot40((uint64_t)bits32, wbase);
+    if (bits)
+      __hip_atomic_fetch_or(a, bits, __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// clock += W after a pull window (one thread; every workgroup of the window
+// has read the clock by then)
+__global__ void k_tick(int64_t* clock, int W) { *clock += W; }
+
+// Re-mask the last step's record of each listed neuron-trial with its
+// current transmit flag (one thread per entry; atomics, since entries of
+// one record word may come from different workgroups).
+__global__ __launch_bounds__(kBlock) void k_pull_resend(fis_sim s,
+                                                        const int32_t* idx,
+                                                        int n_idx) {
+  const int e = (int)(blockIdx.x * kBlock + threadIdx.x);
+  if (e >= n_idx) return;
+  const int src = idx[e];
+  if (src < 0 || src >= s.n * s.bstride) return;
+  if (!(s.st[src] & FIS_ST_SPIKE)) return;
+  const int pre = src / s.bstride;
+  const int b = src - pre * s.bstride;
+  const int words = s.bstride >> 6;
+  const int slot = (int)((*s.clock + kRec - 1) % kRec);
+  uint64_t* w = s.rec + ((int64_t)pre * kRec + slot) * words + (b >> 6);
+  const uint64_t bit = 1ull << (b & 63);
+  if (s.fl[src] & FIS_OUT_OFF) {
+    __hip_atomic_fetch_and(w, ~bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    __hip_atomic_fetch_or(w, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_or(s.act + pre, 1ull << slot, __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// The input in flight in the records, added into a push-road ring: one
+// wave per (neuron, step u) of the last 20 steps (blockIdx.y = u - clock
+// + 20), rows by presynaptic neuron.
+__global__ __launch_bounds__(kBlock) void k_pull_to_ring(fis_sim s,
+                                                         int32_t* ring) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int pre = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
+  if (pre >= s.n) return;
+  const int64_t u = *s.clock - 20 + (int64_t)blockIdx.y;
+  if (u < 0) return;
+  const int slot = (int)(u % kRec);
+  if (!((s.act[pre] >> slot) & 1ull)) return;
+  const int words = s.bstride >> 6;
+  const int64_t nb = (int64_t)s.n * s.bstride;
+  int32_t* base = ring + (int64_t)((u + 1) % FIS_RING_LEN) * nb;
+  for (int w = 0; w < words; ++w) {
+    const uint64_t m = s.rec[((int64_t)pre * kRec + slot) * words + w];
+    if (m == 0) continue;
+    for (int64_t k = s.offsets[pre] + lane; k < s.offsets[pre + 1];
+         k += kWave) {
+      int32_t* p = base + (int64_t)s.cols[k] * s.bstride + (w << 6);
+      const int32_t val = s.vals[k];
+      uint64_t mm = m;
+      while (mm) {
+        const int j = __ffsll((long long)mm) - 1;
+        mm &= mm - 1ull;
+        __hip_atomic_fetch_add(p + j, val, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------- slots
+
+// One thread per neuron; it resets every listed trial of its neuron, so a
+// workgroup writes only the rows of its own 256 neurons (whole lines).
+__global__ __launch_bounds__(kBlock) void k_reset_trials(fis_sim s,
+                                                         const int32_t* trials,
+                                                         int n_trials,
+                                                         float v0) {
+  const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
+  if (i >= s.n) return;
+  const int bs = s.bstride;
+  const int64_t nb = (int64_t)s.n * bs;
+  const int words = bs >> 6;
+  for (int q = 0; q < n_trials; ++q) {
+    const int b = trials[q];
+    if (b < 0 || b >= s.batch) continue;
+    const int idx = i * bs + b;
+    s.v[idx] = v0;
+    s.g[idx] = 0.0f;
+    s.st[idx] = (uint8_t)min(s.refrac_steps, (int)FIS_ST_RMAX);
+    s.fl[idx] = 0;
+    s.sbits[idx] = 0u;
+    if (s.vhold) s.vhold[idx] = 0.0f;
+    if (s.counts) s.counts[idx] = 0u;
+    if (s.spikes_out) s.spikes_out[idx] = 0.0f;
+    for (int k = 0; k < FIS_RING_LEN; ++k) s.ring[k * nb + idx] = 0;
+    if (s.rec && words > 0) {
+      uint64_t* r = s.rec + (int64_t)i * kRec * words + (b >> 6);
+      const uint64_t keep = ~(1ull << (b & 63));
+      for (int k = 0; k < kRec; ++k) r[k * words] &= keep;
+    }
+  }
+}
+
+// Byte-wise flag set through the containing 32-bit word (entries of one
+// word may come from different workgroups).
+__global__ __launch_bounds__(kBlock) void k_flag_pairs(fis_sim s,
+                                                       const int32_t* idx,
+                                                       int n, uint32_t bits) {
+  const int e = (int)(blockIdx.x * kBlock + threadIdx.x);
+  if (e >= n) return;
+  const int x = idx[e];
+  if (x < 0 || x >= s.n * s.bstride) return;
+  const int sh = (x & 3) * 8;
+  atomicOr((uint32_t*)(s.fl + (x & ~3)), (bits & 0xFFu) << sh);
+  if (bits & FIS_NOREFRAC)
+    atomicAnd((uint32_t*)(s.st + (x & ~3)), ~(0xFFu << sh));
+}
+
 // ------------------------------------------------------------- ev_spmv twin
 
 // out[b, post] += sum_pre W[post, pre] * sp[b, pre] * mo[b, pre].  Each lane
@@ -484,6 +835,16 @@ int check_sim(const fis_sim* s) {
   return 0;
 }
 
+int check_pull(const fis_sim* s) {
+  if (int rc = check_sim(s)) return rc;
+  const int bs = s->bstride;
+  if (bs < kWave || bs > kPull || (bs & (bs - 1)) != 0)
+    return (int)hipErrorInvalidValue;
+  if (!s->in_offsets || !s->in_cols || !s->in_vals || !s->rec || !s->act)
+    return (int)hipErrorInvalidValue;
+  return 0;
+}
+
 unsigned grid_for(int64_t items) {
   return (unsigned)((items + kBlock - 1) / kBlock);
 }
@@ -505,6 +866,8 @@ int64_t fis_struct_layout(int what) {
     case 4: return (int64_t)offsetof(fis_sim, raster_len);
     case 5: return (int64_t)sizeof(fis_event);
     case 6: return (int64_t)sizeof(fis_drive_entry);
+    case 7: return (int64_t)offsetof(fis_sim, in_offsets);
+    case 8: return (int64_t)offsetof(fis_sim, t_origin);
     default: return -1;
   }
 }
@@ -614,6 +977,72 @@ int fis_seed_ring(const fis_sim* s, int slot, void* stream) {
   const int64_t nb = (int64_t)s->n * s->bstride;
   hipLaunchKernelGGL(k_seed_ring, dim3(grid_for(nb)), dim3(kBlock), 0,
                      (hipStream_t)stream, *s, slot);
+  return (int)hipGetLastError();
+}
+
+int fis_resend(const fis_sim* s, const int32_t* idx, int n_idx, int slot,
+               void* stream) {
+  if (int rc = check_sim(s)) return rc;
+  if (slot < 0 || slot >= FIS_RING_LEN || n_idx < 0 || (n_idx > 0 && !idx))
+    return (int)hipErrorInvalidValue;
+  if (n_idx == 0) return 0;
+  const int64_t threads = (int64_t)n_idx * kWave;
+  hipLaunchKernelGGL(k_resend, dim3(grid_for(threads)), dim3(kBlock), 0,
+                     (hipStream_t)stream, *s, idx, n_idx, slot);
+  return (int)hipGetLastError();
+}
+
+int fis_pull_window(const fis_sim* s, int W, int ring_steps, void* stream) {
+  if (int rc = check_pull(s)) return rc;
+  if (W < 1 || W > FIS_MAX_WINDOW || ring_steps < 0)
+    return (int)hipErrorInvalidValue;
+  const int64_t nb = (int64_t)s->n * s->bstride;
+  const unsigned blocks = (unsigned)((nb + kPull - 1) / kPull);
+  hipLaunchKernelGGL(k_pull<FIS_MAX_WINDOW>, dim3(blocks), dim3(kPull), 0,
+                     (hipStream_t)stream, *s, W, ring_steps);
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, (hipStream_t)stream,
+                     s->clock, W);
+  return (int)hipGetLastError();
+}
+
+int fis_pull_resend(const fis_sim* s, const int32_t* idx, int n_idx,
+                    void* stream) {
+  if (int rc = check_pull(s)) return rc;
+  if (n_idx < 0 || (n_idx > 0 && !idx)) return (int)hipErrorInvalidValue;
+  if (n_idx == 0) return 0;
+  hipLaunchKernelGGL(k_pull_resend, dim3(grid_for(n_idx)), dim3(kBlock), 0,
+                     (hipStream_t)stream, *s, idx, n_idx);
+  return (int)hipGetLastError();
+}
+
+int fis_pull_to_ring(const fis_sim* s, int32_t* ring, void* stream) {
+  if (int rc = check_pull(s)) return rc;
+  if (!ring) return (int)hipErrorInvalidValue;
+  const int64_t threads = (int64_t)s->n * kWave;
+  hipLaunchKernelGGL(k_pull_to_ring, dim3(grid_for(threads), 20), dim3(kBlock),
+                     0, (hipStream_t)stream, *s, ring);
+  return (int)hipGetLastError();
+}
+
+int fis_reset_trials(const fis_sim* s, const int32_t* trials, int n_trials,
+                     float v0, void* stream) {
+  if (int rc = check_sim(s)) return rc;
+  if (n_trials < 0 || (n_trials > 0 && !trials))
+    return (int)hipErrorInvalidValue;
+  if (n_trials == 0) return 0;
+  hipLaunchKernelGGL(k_reset_trials, dim3(grid_for(s->n)), dim3(kBlock), 0,
+                     (hipStream_t)stream, *s, trials, n_trials, v0);
+  return (int)hipGetLastError();
+}
+
+int fis_flag_pairs(const fis_sim* s, const int32_t* idx, int n, uint32_t bits,
+                   void* stream) {
+  if (int rc = check_sim(s)) return rc;
+  if (n < 0 || (n > 0 && !idx) || bits > 0xFFu)
+    return (int)hipErrorInvalidValue;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_flag_pairs, dim3(grid_for(n)), dim3(kBlock), 0,
+                     (hipStream_t)stream, *s, idx, n, bits);
   return (int)hipGetLastError();
 }
 

@@ -20,12 +20,13 @@ CSRC = PKG.parent / "csrc"
 SOURCES = [CSRC / "lif.hip"]
 HEADER = ROOT / "include" / "fis.h"
 LIB_PATH = PKG / "libfis.so"
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 MAX_WINDOW = 20
 RING_LEN = 20
 FIRE_OFF, OUT_OFF, IN_OFF, HOLD, NOREFRAC = 0x01, 0x02, 0x04, 0x08, 0x10
 BLOCK_WAVES, LINE_WORDS, LINE_INTS, EVENT_ALIGN = 4, 16, 32, 8
+REC_SLOTS, PULL_TILE = 40, 1024
 ST_SPIKE, ST_RMAX = 0x80, 127
 
 HIPCC_FLAGS = [
@@ -81,6 +82,9 @@ class FisSim(ctypes.Structure):
         ("refrac_steps", ctypes.c_int32),
         ("spikes_out", P), ("counts", P), ("pop_counts", P),
         ("raster", P), ("raster_len", ctypes.c_int32),
+        ("in_offsets", P), ("in_cols", P), ("in_vals", P),
+        ("rec", P), ("act", P),
+        ("t_origin", P), ("t_end", P),
     ]
 
 
@@ -123,6 +127,12 @@ def lib():
         "fis_update": ([sims, I, P], I),
         "fis_scatter": ([sims, I, P], I),
         "fis_seed_ring": ([sims, I, P], I),
+        "fis_resend": ([sims, P, I, I, P], I),
+        "fis_pull_window": ([sims, I, I, P], I),
+        "fis_pull_resend": ([sims, P, I, P], I),
+        "fis_pull_to_ring": ([sims, P, P], I),
+        "fis_reset_trials": ([sims, P, I, ctypes.c_float, P], I),
+        "fis_flag_pairs": ([sims, P, I, ctypes.c_uint32, P], I),
     }
     for name, (args, res) in sig.items():
         try:
@@ -137,7 +147,8 @@ def lib():
     want = {0: ctypes.sizeof(FisSim), 1: FisSim.offsets.offset,
             2: FisSim.w_scale.offset, 3: FisSim.refrac_steps.offset,
             4: FisSim.raster_len.offset, 5: EVENT_BYTES,
-            6: ctypes.sizeof(FisDriveEntry)}
+            6: ctypes.sizeof(FisDriveEntry), 7: FisSim.in_offsets.offset,
+            8: FisSim.t_origin.offset}
     for k, v in want.items():
         got = L.fis_struct_layout(k)
         if got != v:

@@ -7,8 +7,10 @@ own Philox seed, drive table entries and m_out masks; trials never
 interact, so trial ``b`` of a batch produces the same raster as the same
 trial run alone (tests check this bit for bit).
 
-One window is three launches -- drive table, update (W steps), scatter --
-and ``graph_windows`` windows are captured into one graph.  The step
+One window is the drive-table launch plus the road's launches: on the
+push road update (W steps) and scatter, on the pull road (64..1,024
+trials, the default there) one fused gather+update launch.
+``graph_windows`` windows are captured into one graph.  The step
 counter lives in device memory, so the same graph serves every replay.
 """
 
@@ -24,7 +26,8 @@ from .workloads import Workload
 
 class BatchRunner:
     def __init__(self, net: DeviceNet, wl: Workload, params=None,
-                 raster_len=0, counts=True, pop_counts=False, window=20):
+                 raster_len=0, counts=True, pop_counts=False, window=20,
+                 road="auto"):
         if not 1 <= window <= L.MAX_WINDOW:
             raise ValueError("window must be 1..%d" % L.MAX_WINDOW)
         self.wl = wl
@@ -32,7 +35,7 @@ class BatchRunner:
         sim = DeviceSim(net, wl.batch, params=params, seeds=wl.seeds,
                         trial_ids=np.zeros(wl.batch, np.uint32),
                         counts=counts, pop_counts=pop_counts,
-                        raster_len=raster_len)
+                        raster_len=raster_len, road=road)
         self.sim = sim
         # per-trial refractory exemption of driven neurons; the counter
         # starts at refrac_steps = 0 for them (sim.py:371-388)
@@ -54,8 +57,7 @@ class BatchRunner:
         s = self.sim
         if self.table is not None:
             s.drive_table(self.table, W)
-        s.update(W)
-        s.scatter()
+        s.step_launch(W)
 
     def capture(self, graph_windows=50):
         """Capture ``graph_windows`` full windows into one HIP graph.
@@ -65,7 +67,7 @@ class BatchRunner:
         """
         s = self.sim
         names = ["v", "g", "st", "ring", "sbits", "ev_count", "clock"]
-        names += [k for k in ("counts", "pop_counts", "raster")
+        names += [k for k in ("counts", "pop_counts", "raster", "rec", "act")
                   if getattr(s, k) is not None]
         saved = {k: getattr(s, k).clone() for k in names}
         t0 = s.t
@@ -102,7 +104,7 @@ class BatchRunner:
                 left -= per
         while left > 0:
             w = min(W, left)
-            self._window(w)  # update() advances the host clock
+            self._window(w)  # the road's launch advances the host clock
             left -= w
 
     def counts(self):

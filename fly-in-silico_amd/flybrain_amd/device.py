@@ -23,10 +23,33 @@ from . import _lib as L
 from .params import derived
 
 
-def stride_for(batch: int) -> int:
-    """Trial stride: next power of two below 64, next multiple of 64 above."""
+ROADS = ("push", "pull")
+NO_END = 1 << 62  # t_end of a trial that never stops counting
+PULL_MIN_BATCH = 64
+
+
+def pick_road(batch: int, road: str = "auto") -> str:
+    """``auto``: the pull road from 64 trials up (to 1,024), else push."""
+    if road == "auto":
+        return "pull" if PULL_MIN_BATCH <= batch <= L.PULL_TILE else "push"
+    if road not in ROADS:
+        raise ValueError("road %r not in %s" % (road, ROADS))
+    if road == "pull" and batch > L.PULL_TILE:
+        raise ValueError("the pull road holds at most %d trials"
+                         % L.PULL_TILE)
+    return road
+
+
+def stride_for(batch: int, road: str = "push") -> int:
+    """Trial stride.  Push road: next power of two below 64, next multiple
+    of 64 above.  Pull road: next power of two, at least 64."""
     if batch < 1:
         raise ValueError("batch must be >= 1")
+    if road == "pull":
+        p = 64
+        while p < batch:
+            p *= 2
+        return p
     if batch >= 64:
         return (batch + 63) // 64 * 64
     p = 1
@@ -56,6 +79,25 @@ class DeviceNet:
         self.offsets = torch.from_numpy(conn.offsets).to(self.device)
         self.cols = torch.from_numpy(conn.cols).to(self.device)
         self.vals = torch.from_numpy(conn.vals).to(self.device)
+        self._in = None
+
+    def in_csr(self):
+        """The same synapses by postsynaptic neuron (W[post, pre] CSR,
+        int32 offsets), built once on first use by the pull road."""
+        if self._in is None:
+            c = self.conn
+            if c.n_edges >= (1 << 31):
+                raise ValueError("the pull road indexes edges with int32")
+            pre = np.repeat(np.arange(c.n, dtype=np.int32),
+                            np.diff(c.offsets))
+            order = np.argsort(c.cols, kind="stable")
+            off = np.zeros(c.n + 1, np.int64)
+            np.cumsum(np.bincount(c.cols, minlength=c.n), out=off[1:])
+            dev = self.device
+            self._in = (torch.from_numpy(off.astype(np.int32)).to(dev),
+                        torch.from_numpy(pre[order]).to(dev),
+                        torch.from_numpy(c.vals[order]).to(dev))
+        return self._in
 
     @property
     def n_edges(self):
@@ -67,8 +109,9 @@ class DeviceSim:
 
     def __init__(self, net: DeviceNet, batch: int, params=None, seeds=None,
                  trial_ids=None, no_refractory=None, spikes_out=False,
-                 counts=False, pop_counts=False, raster_len=0):
+                 counts=False, pop_counts=False, raster_len=0, road="push"):
         self.lib = L.lib()
+        self.road = pick_road(int(batch), road)
         if net.device.type != "cuda":
             raise L.KernelUnavailable("the engine runs on a GPU only; got "
                                       "device %r" % (net.device,))
@@ -76,7 +119,7 @@ class DeviceSim:
         self.c = derived(params)
         self.n = n = net.n
         self.batch = B = int(batch)
-        self.bstride = bs = stride_for(B)
+        self.bstride = bs = stride_for(B, self.road)
         self.nb = nb = n * bs
         self.n_waves = int(self.lib.fis_n_waves(n, bs))
         self.n_blocks = int(self.lib.fis_n_blocks(n, bs))
@@ -89,8 +132,10 @@ class DeviceSim:
         z = dict(device=dev)
         self.v = torch.full((nb,), float(self.c["v0"]), dtype=torch.float32, **z)
         self.g = torch.zeros(nb, dtype=torch.float32, **z)
-        self.st = torch.zeros(nb, dtype=torch.uint8, **z)
-        self.fl = torch.zeros(nb, dtype=torch.uint8, **z)
+        # byte arrays padded to whole 32-bit words (fis_flag_pairs)
+        nb4 = (nb + 3) // 4 * 4
+        self.st = torch.zeros(nb4, dtype=torch.uint8, **z)[:nb]
+        self.fl = torch.zeros(nb4, dtype=torch.uint8, **z)[:nb]
         # allocated up front: a captured graph holds this pointer
         self.vhold = torch.zeros(nb, dtype=torch.float32, **z)
         self.ring = torch.zeros(L.RING_LEN * nb, dtype=torch.int32, **z)
@@ -109,6 +154,11 @@ class DeviceSim:
                                                                   np.uint32)
         self.trial_ids_host = ti
         self.trial_ids = torch.from_numpy(ti.view(np.int32)).to(dev)
+        # per-trial clock: origin 0, no end, until a slot is (re)started
+        self.t_origin_host = np.zeros(bs, np.int64)
+        self.t_end_host = np.full(bs, NO_END, np.int64)
+        self.t_origin = torch.from_numpy(self.t_origin_host).to(dev)
+        self.t_end = torch.from_numpy(self.t_end_host).to(dev)
         self.spikes_out = (torch.zeros(nb, dtype=torch.float32, **z)
                            if spikes_out else None)
         self.counts = torch.zeros(nb, dtype=torch.int32, **z) if counts else None
@@ -120,8 +170,17 @@ class DeviceSim:
         self.raster = (torch.zeros(self.raster_len * self.n_blocks
                                    * L.LINE_WORDS, dtype=torch.int64, **z)
                        if raster_len else None)
+        if self.road == "pull":
+            self.in_offsets, self.in_cols, self.in_vals = net.in_csr()
+            self.rec = torch.zeros(L.REC_SLOTS * n * (bs // 64),
+                                   dtype=torch.int64, **z)
+            self.act = torch.zeros(n, dtype=torch.int64, **z)
+        else:
+            self.in_offsets = self.in_cols = self.in_vals = None
+            self.rec = self.act = None
         self.t = 0  # host mirror of the device clock
         self._open_window = 0
+        self.ring_until = 0  # pull road: steps < this also read the ring
 
         # Host mirrors of the host-written per-neuron-trial arrays.  Changes
         # are made here and sent down as one contiguous copy: scattered
@@ -150,7 +209,8 @@ class DeviceSim:
         s.cols = self.net.cols.data_ptr()
         s.vals = self.net.vals.data_ptr()
         for name in ("v", "g", "st", "fl", "ring", "sbits", "events",
-                     "ev_count", "clock", "seeds", "trial_ids"):
+                     "ev_count", "clock", "seeds", "trial_ids", "t_origin",
+                     "t_end"):
             setattr(s, name, getattr(self, name).data_ptr())
         s.vhold = self.vhold.data_ptr() if self.vhold is not None else None
         for name in ("w_scale", "a_syn", "a_mem", "v_rest", "v_reset", "v_th",
@@ -161,6 +221,9 @@ class DeviceSim:
             t = getattr(self, name)
             setattr(s, name, t.data_ptr() if t is not None else None)
         s.raster_len = self.raster_len
+        for name in ("in_offsets", "in_cols", "in_vals", "rec", "act"):
+            t = getattr(self, name)
+            setattr(s, name, t.data_ptr() if t is not None else None)
         self.struct = s
         self._sp = ctypes.byref(s)
         self.version = getattr(self, "version", 0) + 1
@@ -188,10 +251,13 @@ class DeviceSim:
         tr = self._trials(trials)
         ne = self._neurons(neurons)
         f = self.fl_host
+        old = f.copy() if bit & L.OUT_OFF else None
         f[:, tr] &= np.uint8(0xFF ^ bit)
         if ne.size:
             f[np.ix_(ne, tr)] |= np.uint8(bit)
         self._push_flags()
+        if old is not None:
+            self._resend(old)
 
     def set_flag_pairs(self, bit, neurons, trials):
         """Set ``bit`` on explicit (neuron, trial) pairs (other entries kept)."""
@@ -202,8 +268,29 @@ class DeviceSim:
         if tr.size and (tr.min() < 0 or tr.max() >= self.batch):
             raise IndexError("trial index outside [0, %d)" % self.batch)
         if ne.size:
+            old = self.fl_host.copy() if bit & L.OUT_OFF else None
             self.fl_host[ne, tr] |= np.uint8(bit)
             self._push_flags()
+            if old is not None:
+                self._resend(old)
+
+    def _resend(self, old):
+        """The last step's spikes were scattered with the old transmit flag;
+        the reference masks them with the flag current at the next step
+        (sim.py:910-918).  Correct the ring slot they went to."""
+        changed = np.flatnonzero(((old ^ self.fl_host) & L.OUT_OFF).reshape(-1))
+        if changed.size == 0:
+            return
+        idx = torch.from_numpy(changed.astype(np.int32)).to(self.device)
+        if self.road == "pull":
+            L.check(self.lib.fis_pull_resend(self._sp, _ptr(idx),
+                                             int(idx.numel()), _stream()),
+                    "fis_pull_resend")
+        else:
+            L.check(self.lib.fis_resend(self._sp, _ptr(idx), int(idx.numel()),
+                                        self.t % L.RING_LEN, _stream()),
+                    "fis_resend")
+        self._resend_idx = idx  # kept alive until the launch has run
 
     def exempt_pairs(self, neurons, trials):
         """Refractory period 0 for (neuron, trial) pairs, before the first
@@ -219,6 +306,65 @@ class DeviceSim:
             st = self.st.cpu().numpy().reshape(self.n, self.bstride)
             st[ne, tr] = 0
             self.st.copy_(torch.from_numpy(st.reshape(-1)))
+
+    # ------------------------------------------------------------ slots
+    def restart_trials(self, trials, seeds, t_end, out_neurons=(),
+                       out_trials=(), free_neurons=(), free_trials=()):
+        """Start fresh trials in slots ``trials`` at the current clock.
+
+        Their state is reset on the device (fis_reset_trials: the state
+        Engine.__init__ gives, nothing in flight), their Philox key is
+        ``seeds``, their own clock starts at 0 now and their spikes are
+        counted until global step ``t_end``; (out_neurons, out_trials)
+        pairs get m_out = 0 and (free_neurons, free_trials) pairs are
+        refractory-exempt.  Other trials are not touched.
+        """
+        tr = self._trials(trials)
+        if tr.size == 0:
+            return
+        sd = np.asarray(seeds, np.uint64).reshape(-1)
+        te = np.asarray(t_end, np.int64).reshape(-1)
+        if sd.size != tr.size or te.size != tr.size:
+            raise ValueError("one seed and one end step per trial")
+        dev = self.device
+        tt = torch.from_numpy(tr.astype(np.int32)).to(dev)
+        L.check(self.lib.fis_reset_trials(self._sp, _ptr(tt), int(tr.size),
+                                          float(self.c["v0"]), _stream()),
+                "fis_reset_trials")
+        self.fl_host[:, tr] = 0
+        self.vhold_host[:, tr] = 0.0
+        keep = [tt]
+        for bit, ne, tb in ((L.OUT_OFF, out_neurons, out_trials),
+                            (L.NOREFRAC, free_neurons, free_trials)):
+            ne = self._neurons(ne)
+            tb = np.asarray(tb, np.int64).reshape(-1)
+            if ne.size != tb.size:
+                raise ValueError("neurons and trials differ in length")
+            if ne.size == 0:
+                continue
+            if not np.isin(tb, tr).all():
+                raise ValueError("flag pairs must lie in the restarted trials")
+            self.fl_host[ne, tb] |= np.uint8(bit)
+            idx = torch.from_numpy((ne * self.bstride + tb).astype(np.int32)
+                                   ).to(dev)
+            L.check(self.lib.fis_flag_pairs(self._sp, _ptr(idx),
+                                            int(idx.numel()), bit, _stream()),
+                    "fis_flag_pairs")
+            keep.append(idx)
+        self.seeds_host[tr] = sd
+        self.seeds.copy_(torch.from_numpy(self.seeds_host.view(np.int64)))
+        self.t_origin_host[tr] = self.t
+        self.t_end_host[tr] = te
+        self.t_origin.copy_(torch.from_numpy(self.t_origin_host))
+        self.t_end.copy_(torch.from_numpy(self.t_end_host))
+        self._slot_keep = keep  # index buffers live until the launches ran
+
+    def trial_counts(self, trial):
+        """Spike counts of one trial, int64 [n] (one column, on the host)."""
+        if self.counts is None:
+            raise RuntimeError("built without counts")
+        col = self.counts.view(self.n, self.bstride)[:, int(trial)]
+        return col.to(torch.int64).cpu().numpy()
 
     def set_hold(self, neurons, v_clamp, trials=None):
         tr = self._trials(trials)
@@ -259,14 +405,36 @@ class DeviceSim:
 
     def update(self, W):
         """W update steps; the window closes with :meth:`scatter`."""
+        self._push_only("update")
         L.check(self.lib.fis_update(self._sp, W, _stream()), "fis_update")
         self._open_window = W
 
     def scatter(self):
         """Scatter the window's spikes and advance the clock by W."""
+        self._push_only("scatter")
         W = self._open_window
         L.check(self.lib.fis_scatter(self._sp, W, _stream()), "fis_scatter")
         self.t += W
+
+    def _push_only(self, what):
+        if self.road != "push":
+            raise RuntimeError("%s is a push-road launch; this simulation "
+                               "runs the pull road (use window/pull)" % what)
+
+    def pull(self, W):
+        """One pull-road window of W steps: gather + update + clock += W."""
+        ring_steps = max(0, min(W, self.ring_until - self.t))
+        L.check(self.lib.fis_pull_window(self._sp, W, ring_steps, _stream()),
+                "fis_pull_window")
+        self.t += W
+
+    def step_launch(self, W):
+        """The road's launches of one window after its drive."""
+        if self.road == "pull":
+            self.pull(W)
+        else:
+            self.update(W)
+            self.scatter()
 
     def window(self, W, drive=None):
         """One window: drive (a DriveTable, Tape or rates tensor), update,
@@ -282,8 +450,7 @@ class DeviceSim:
                 self.drive_tape(drive, W)
             else:
                 raise TypeError("unknown drive %r" % type(drive))
-        self.update(W)
-        self.scatter()
+        self.step_launch(W)
 
     def seed_ring(self, slot):
         L.check(self.lib.fis_seed_ring(self._sp, slot, _stream()),
@@ -300,12 +467,25 @@ class DeviceSim:
     def spike_state(self):
         return (self.view(self.st) & L.ST_SPIKE) != 0
 
+    def ring_image(self):
+        """The input in flight as the push road's int32 ring
+        [20 * n * bstride] (spikes of step u in slot (u+1) mod 20): the
+        ring itself on the push road; on the pull road the carried-over
+        ring plus the spike records of the last 20 steps scattered into a
+        copy (fis_pull_to_ring)."""
+        if self.road == "push":
+            return self.ring
+        img = self.ring.clone()
+        L.check(self.lib.fis_pull_to_ring(self._sp, _ptr(img), _stream()),
+                "fis_pull_to_ring")
+        return img
+
     def ring_counts(self, step_input):
         """Integer synaptic input that step ``step_input`` reads, i.e.
         the sum computed from the spikes of step ``step_input - 20``;
         [batch, n]."""
         slot = (step_input + 1) % L.RING_LEN
-        return self.view(self.ring[slot * self.nb:(slot + 1) * self.nb])
+        return self.view(self.ring_image()[slot * self.nb:(slot + 1) * self.nb])
 
     def read_raster(self, t_from, t_to):
         """Spikes of steps [t_from, t_to) from the raster ring, bool
@@ -327,7 +507,7 @@ class DeviceSim:
     def rec_slot(self, u):
         """The reference's rec(u) (sum of spikes of step u-1), int [B, n]."""
         slot = u % L.RING_LEN
-        return self.view(self.ring[slot * self.nb:(slot + 1) * self.nb])
+        return self.view(self.ring_image()[slot * self.nb:(slot + 1) * self.nb])
 
 
 class DriveTable:

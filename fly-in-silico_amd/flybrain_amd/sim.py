@@ -267,7 +267,8 @@ class Engine:
         (assembled on the host from one copy of the int32 ring)."""
         s = self._sim
         T = s.t
-        ring = s.ring.cpu().numpy().reshape(L.RING_LEN, s.n, s.bstride)
+        ring = s.ring_image().cpu().numpy().reshape(L.RING_LEN, s.n,
+                                                    s.bstride)
         w = np.float32(self.w_scale)
         out = np.zeros((self.batch, self.buf_len, self.n), np.float32)
         for u in range(max(0, T - self.buf_len), T):

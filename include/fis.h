@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define FIS_ABI_VERSION 1
+#define FIS_ABI_VERSION 2
 #define FIS_MAX_WINDOW 20   /* steps per window: delay horizon 19 + 1     */
 #define FIS_RING_LEN 20     /* int32 ring slots: rec(t-19) .. rec(t)       */
 
@@ -47,6 +47,8 @@ extern "C" {
 #define FIS_LINE_WORDS 16   /* 128 bytes = 16 x 8-byte words              */
 #define FIS_LINE_INTS 32    /* 128 bytes = 32 x 4-byte ints               */
 #define FIS_EVENT_ALIGN 8   /* 8 events = 128 bytes                       */
+#define FIS_REC_SLOTS 40    /* pull road: spike records of steps t mod 40 */
+#define FIS_PULL_TILE 1024  /* pull road: neuron-trials per workgroup     */
 
 /* flag bits of `fl`, one byte per neuron-trial (sim.py:393-410, :866-900) */
 #define FIS_FIRE_OFF 0x01u  /* m_fire == 0: spike output suppressed        */
@@ -90,8 +92,9 @@ typedef struct {
   float* v;
   float* g;
   uint8_t* st;
-  const uint8_t* fl;
-  const float* vhold;     /* nullable; read where FIS_HOLD is set      */
+  uint8_t* fl;            /* written only by fis_reset_trials and
+                             fis_flag_pairs                            */
+  float* vhold;           /* nullable; read where FIS_HOLD is set      */
   int32_t* ring;          /* [FIS_RING_LEN][n * bstride]               */
   uint32_t* sbits;        /* [n * bstride] kick bits of the window     */
   fis_event* events;      /* [n_waves * ev_cap]                        */
@@ -121,6 +124,23 @@ typedef struct {
                            at step t are bit j (lane j) of word
                            [t mod raster_len][w / 4][w % 4]             */
   int32_t raster_len;   /* >= FIS_MAX_WINDOW                           */
+  /* pull road only (nullable otherwise): the connectome by postsynaptic
+   * neuron (W CSR, the reference's own W[post, pre] orientation) and the
+   * spike records of the last 40 steps.  rec word
+   * [pre][t mod 40][w] bit j = trial 64 w + j of `pre` transmitted a spike
+   * at step t; act[pre] bit (t mod 40) = some trial of `pre` did.         */
+  const int32_t* in_offsets; /* [n+1]                                  */
+  const int32_t* in_cols;    /* [E] presynaptic neuron                 */
+  const int16_t* in_vals;    /* [E] signed synapse count               */
+  uint64_t* rec;             /* [n][40][bstride / 64]                  */
+  uint64_t* act;             /* [n]                                    */
+  /* per-trial clock (nullable = 0 and never): trial b's own step is
+   * clock - t_origin[b] -- the Philox counter and the drive table's rate
+   * segment use it, so a trial's drive does not depend on when its slot
+   * started -- and its spikes are counted only at global steps
+   * < t_end[b].  [bstride] each. */
+  const int64_t* t_origin;
+  const int64_t* t_end;
 } fis_sim;
 
 /* Drive table entry: neuron-trial `idx` is driven at rate
@@ -181,9 +201,62 @@ int fis_scatter(const fis_sim* s, int W, void* stream);
  * slot `slot` -- used after a restore to rebuild the input in flight. */
 int fis_seed_ring(const fis_sim* s, int slot, void* stream);
 
+/* After FIS_OUT_OFF changed on the neuron-trials idx[0..n_idx) between two
+ * steps: the reference masks the PREVIOUS step's spikes with the CURRENT
+ * m_out (rec = W @ (spikes * m_out), sim.py:910-918, masks written between
+ * steps by set_masks, sim.py:523-571), while this engine scattered them at
+ * the end of that step with the old flag.  For each listed neuron-trial
+ * whose last step spiked (st bit 7), its row is added into ring slot `slot`
+ * (flag now clear) or subtracted (flag now set).  fl must already hold the
+ * new flags; `slot` is the slot that step's scatter wrote, clock mod 20. */
+int fis_resend(const fis_sim* s, const int32_t* idx, int n_idx, int slot,
+               void* stream);
+
+/* ---- pull road: one fused launch per window (bstride a power of two,
+ * 64 .. 1024).  Each workgroup owns 1024 neuron-trials (1024 / bstride
+ * neurons x all trials).  It sums the input of its neurons for the W steps
+ * t0 .. t0+W-1 from the spike records of steps t0-20 .. t0+W-21 over their
+ * in-edges into LDS (exact int32), then runs the W update steps of
+ * Engine._core (sim.py:920-953) from LDS and writes the window's own
+ * spike records; finally clock += W.  The int32 ring is not used, except
+ * that the first `ring_steps` steps of the window also consume (and clear)
+ * ring slot (t+1) mod 20, which holds input carried over from a restore. */
+int fis_pull_window(const fis_sim* s, int W, int ring_steps, void* stream);
+
+/* Pull-road counterpart of fis_resend: after FIS_OUT_OFF changed on the
+ * listed neuron-trials, re-mask the record of the last step (clock - 1)
+ * with the new flags (sim.py:910-918 masks the previous step's spikes). */
+int fis_pull_resend(const fis_sim* s, const int32_t* idx, int n_idx,
+                    void* stream);
+
+/* Add the input still in flight in the pull road's records (the spikes of
+ * steps clock-20 .. clock-1) into `ring` ([FIS_RING_LEN][n * bstride],
+ * the push road's layout: spikes of step u land in slot (u+1) mod 20),
+ * using the W^T CSR -- for state views and snapshots. */
+int fis_pull_to_ring(const fis_sim* s, int32_t* ring, void* stream);
+
+/* ---- slots (continuous batching) --------------------------------------
+ * Reset trials trials[0..n_trials) to the state Engine.__init__ gives
+ * (sim.py:371-410): v = v0, g = 0, refractory counter = refrac_steps, no
+ * spike, all flags clear, no kicks, counts 0, and nothing in flight for
+ * them (int32 ring column, and on the pull road their record bits).  One
+ * thread per neuron walks the listed trials, so every workgroup writes
+ * whole lines of its own neurons. */
+int fis_reset_trials(const fis_sim* s, const int32_t* trials, int n_trials,
+                     float v0, void* stream);
+
+/* OR flag bits into fl[idx[k]] for k < n (byte atomics on the containing
+ * word); with FIS_NOREFRAC the refractory counter of those neuron-trials is
+ * set to 0 as well (Engine.__init__: refrac = refrac_steps = 0 for driven
+ * neurons, sim.py:371-388).  fl and st must be allocated to a multiple of
+ * 4 bytes. */
+int fis_flag_pairs(const fis_sim* s, const int32_t* idx, int n,
+                   uint32_t bits, void* stream);
+
 /* sizeof / offsetof of the structs above (0: sizeof fis_sim, 1: offsets,
  * 2: w_scale, 3: refrac_steps, 4: raster_len, 5: sizeof fis_event,
- * 6: sizeof fis_drive_entry), so a binding can check its mirror. */
+ * 6: sizeof fis_drive_entry, 7: in_offsets, 8: t_origin), so a binding
+ * can check its mirror. */
 int64_t fis_struct_layout(int what);
 
 /* Number of update waves (event segments) for a given n and bstride. */

@@ -147,3 +147,43 @@ def test_tiny_masks_through_engine(golden):
         assert fp(sp.cpu().numpy()) == z["fps"][t][2], t
         db = eng.delay_buffer.cpu().numpy() + np.float32(0)
         assert fp(db) == z["fps"][t][4], t
+
+
+def test_out_mask_masks_the_previous_steps_spikes():
+    """m_out set (or lifted) between steps applies to the spikes of the step
+    just taken, as rec = W @ (spikes * m_out) in the reference's next step
+    (sim.py:910-918): a spike at step T-1 followed by set_masks(out=...)
+    before step T is not transmitted (and is, when the mask is lifted)."""
+    from flybrain_amd.device import Tape
+
+    conn = C.synthetic(300, None, mean_out=60, count_exp=1.1, n_groups=0,
+                       seed=21)
+    n, B = conn.n, 2
+    hub = int(np.argmax(conn.degree()))
+    kicks = {9: [(hub, 0), (hub, 1)], 39: [(hub, 0), (hub, 1)],
+             69: [(hub, 1)]}
+    steps = 100
+    bits = np.zeros((steps, B, n), np.uint8)
+    for t, pairs in kicks.items():
+        for i, b in pairs:
+            bits[t, b, i] = 1
+    t_, b_, i_ = np.nonzero(bits)
+    eng = S.Engine(conn, n, batch=B)
+    tape = Tape(eng._sim, t_, i_, b_)
+    ora = O.OracleEngine(conn, B)
+    masks = {10: dict(out=[hub]), 40: dict(out=[]), 70: dict(out=[hub])}
+    for t in range(steps):
+        if t in masks:
+            eng.set_masks(**masks[t])
+            ora.set_masks(**masks[t])
+        sp = eng.step_tape(tape)
+        ora.step_bits(bits[t])
+        assert np.array_equal(sp.cpu().numpy(), ora.spikes), t
+        np.testing.assert_array_equal(
+            eng.conductance.cpu().numpy().view(np.uint32),
+            ora.conductance.view(np.uint32), err_msg="t=%d" % t)
+        np.testing.assert_array_equal(eng.v.cpu().numpy().view(np.uint32),
+                                      ora.v.view(np.uint32),
+                                      err_msg="t=%d" % t)
+    # the hub did spike at each kicked step (the test exercised the fix)
+    assert ora.conductance.any()

@@ -18,6 +18,7 @@ import torch
 import oracle as O
 from flybrain_amd import connectome as C
 from flybrain_amd.device import DeviceNet, DeviceSim, DriveTable, Tape
+from flybrain_amd._lib import OUT_OFF as O_OUT_OFF
 from gpu_helpers import (apply_masks_device, compare_state, mask_schedule,
                          state_of, tiny_conn)
 
@@ -43,13 +44,15 @@ def tiny(golden):
     return z, conn, bits
 
 
-def test_tiny_masks_per_step(tiny):
+@pytest.mark.parametrize("road", ["push", "pull"])
+def test_tiny_masks_per_step(tiny, road):
     """W=1 steps, masks switched as in the fixture: every step equals the
     oracle and the reference's own fingerprints of v, g and spikes."""
     z, conn, bits = tiny
     B, steps = int(z["batch"]), int(z["steps"])
     net = DeviceNet(conn)
-    sim = DeviceSim(net, B, no_refractory=z["no_refractory"], raster_len=32)
+    sim = DeviceSim(net, B, no_refractory=z["no_refractory"], raster_len=32,
+                    road=road)
     t_, i_, b_ = tape_of_bits(bits)
     tape = Tape(sim, t_, i_, b_)
     ora = O.OracleEngine(conn, B, no_refractory=z["no_refractory"])
@@ -70,13 +73,15 @@ def test_tiny_masks_per_step(tiny):
                                       err_msg="step %d: synaptic input" % t)
 
 
-def test_tiny_masks_windows(tiny):
+@pytest.mark.parametrize("road", ["push", "pull"])
+def test_tiny_masks_windows(tiny, road):
     """Windows of up to 20 steps between mask changes: raster and state at
     every window end equal the oracle; synaptic input equals the golden."""
     z, conn, bits = tiny
     B, steps = int(z["batch"]), int(z["steps"])
     net = DeviceNet(conn)
-    sim = DeviceSim(net, B, no_refractory=z["no_refractory"], raster_len=64)
+    sim = DeviceSim(net, B, no_refractory=z["no_refractory"], raster_len=64,
+                    road=road)
     t_, i_, b_ = tape_of_bits(bits)
     tape = Tape(sim, t_, i_, b_)
     ora = O.OracleEngine(conn, B, no_refractory=z["no_refractory"])
@@ -129,8 +134,10 @@ def test_small10k_config1(golden):
     np.testing.assert_array_equal(r[0], np.minimum(z["refrac_final"][0], 127))
 
 
-@pytest.mark.parametrize("batch", [3, 64, 130])
-def test_philox_drive_batched(batch):
+@pytest.mark.parametrize("batch,road", [(3, "push"), (64, "push"),
+                                        (130, "push"), (64, "pull"),
+                                        (130, "pull"), (256, "pull")])
+def test_philox_drive_batched(batch, road):
     """Dense-rate Philox drive (Engine.step road) and per-trial drive tables
     against the oracle running the same Philox stream, trial by trial."""
     conn = C.synthetic(512, None, mean_out=40, count_exp=1.2, n_groups=4,
@@ -141,7 +148,7 @@ def test_philox_drive_batched(batch):
     drv = [rng.choice(n, rng.integers(5, 40), replace=False) for _ in range(batch)]
     rate = [float(rng.uniform(50, 900)) for _ in range(batch)]
     net = DeviceNet(conn)
-    sim = DeviceSim(net, batch, seeds=seeds, raster_len=40)
+    sim = DeviceSim(net, batch, seeds=seeds, raster_len=40, road=road)
     ne = np.concatenate(drv)
     tr = np.concatenate([np.full(len(d), b) for b, d in enumerate(drv)])
     sim.exempt_pairs(ne, tr)  # refractory exemption per trial
@@ -159,6 +166,60 @@ def test_philox_drive_batched(batch):
             assert np.array_equal(raster[k], ora.spikes.astype(bool)), \
                 "raster differs at step %d" % (a + k)
     compare_state(sim, ora, "end")
+
+
+@pytest.mark.parametrize("batch", [64, 200])
+def test_pull_road_uneven_windows_and_out_masks(batch):
+    """Pull road with a partial last tile (509 neurons), windows of 1..20
+    steps (records of one delay horizon span several windows) and m_out
+    switched between windows on spiking neurons: raster, state and the
+    in-flight input equal the oracle; the push road gives the same."""
+    conn = C.synthetic(509, None, mean_out=45, count_exp=1.15, n_groups=4,
+                       seed=23)
+    n = conn.n
+    rng = np.random.default_rng(batch + 7)
+    seeds = rng.integers(0, 2**63, size=batch, dtype=np.uint64)
+    drv = [rng.choice(n, rng.integers(10, 60), replace=False)
+           for _ in range(batch)]
+    rate = rng.uniform(100, 1500, size=batch).astype(np.float32)
+    ne = np.concatenate(drv)
+    tr = np.concatenate([np.full(len(d), b) for b, d in enumerate(drv)])
+    net = DeviceNet(conn)
+    sims = {}
+    for road in ("pull", "push"):
+        sim = DeviceSim(net, batch, seeds=seeds, raster_len=40, road=road)
+        sim.exempt_pairs(ne, tr)
+        sims[road] = (sim, DriveTable(sim, ne, tr, tr, rate))
+    ora = O.OracleEngine(conn, batch, seeds=seeds, no_refractory_trials=drv)
+    rates = np.zeros((batch, n), np.float32)
+    for b, d in enumerate(drv):
+        rates[b, d] = rate[b]
+    hubs = np.argsort(-conn.degree())[:40]
+    plan = [20, 7, 13, 1, 19, 20, 5, 20, 11, 9, 20, 3, 17, 20, 20, 2, 18]
+    masks = {2: hubs[:25], 5: hubs[10:], 9: [], 12: hubs[::3]}
+    t = 0
+    for k, W in enumerate(plan):
+        if k in masks:
+            for sim, _ in sims.values():
+                sim.set_flag(O_OUT_OFF, masks[k])
+            ora.set_masks(out=list(masks[k]))
+        for sim, table in sims.values():
+            sim.window(W, table)
+        got = {r: s.read_raster(t, t + W) for r, (s, _) in sims.items()}
+        for j in range(W):
+            ora.step_rates(rates)
+            for r in sims:
+                assert np.array_equal(got[r][j], ora.spikes.astype(bool)), \
+                    "%s road: raster differs at step %d" % (r, t + j)
+        t += W
+        for r, (s, _) in sims.items():
+            compare_state(s, ora, "%s road, step %d" % (r, t))
+        img = {r: s.ring_image().cpu().numpy() for r, (s, _) in sims.items()}
+        np.testing.assert_array_equal(
+            img["pull"].reshape(20, n, -1)[:, :, :batch],
+            img["push"].reshape(20, n, -1)[:, :, :batch],
+            err_msg="input in flight at step %d" % t)
+    assert ora.spikes.any()
 
 
 def test_rates_drive_matches_table():
