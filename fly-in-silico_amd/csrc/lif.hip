@@ -556,13 +556,27 @@ __global__ __launch_bounds__(kPull) void k_pull(fis_sim s, int W,
     off[q] = q <= np ? s.in_offsets[p0 + q] : 0x7fffffff;
   __syncthreads();
 
-  {  // 1. gather; record loads of up to 4 steps are in flight per lane
+  {  // 1. gather; the next in-edge is fetched while this one is summed,
+     // and the record loads of up to 8 steps are in flight per lane
+    constexpr int kIn = 8;
     const int w = tid & (words - 1);
     const int G = kPull / words;
     const int e1 = s.in_offsets[p0 + np];
-    for (int e = off[0] + tid / words; e < e1; e += G) {
-      const int pre = s.in_cols[e];
-      const int32_t val = s.in_vals[e];
+    int e = off[0] + tid / words;
+    int pre = 0;
+    int32_t val = 0;
+    if (e < e1) {
+      pre = s.in_cols[e];
+      val = s.in_vals[e];
+    }
+    while (e < e1) {
+      const int en = e + G;
+      int pre_n = 0;
+      int32_t val_n = 0;
+      if (en < e1) {
+        pre_n = s.in_cols[en];
+        val_n = s.in_vals[en];
+      }
       int pl = 0;
 #pragma unroll
       for (int q = 1; q < kPull / kWave; ++q) pl += e >= off[q] ? 1 : 0;
@@ -570,21 +584,21 @@ __global__ __launch_bounds__(kPull) void k_pull(fis_sim s, int W,
       int32_t* ap = acc + pl * bs + (w << 6);
       const uint64_t* rp = s.rec + (int64_t)pre * pstride + w;
       while (m) {
-        int j[4];
-        uint64_t x[4];
+        int j[kIn];
+        uint64_t x[kIn];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < kIn; ++q) {
           j[q] = m ? __ffsll((long long)m) - 1 : -1;
           m &= m - 1ull;
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < kIn; ++q) {
           int sl = rbase + j[q];
           sl = sl >= kRec ? sl - kRec : sl;
           x[q] = j[q] >= 0 ? rp[sl * words] : 0ull;
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < kIn; ++q) {
           uint64_t bits = x[q];
           while (bits) {
             const int b = __ffsll((long long)bits) - 1;
@@ -593,6 +607,9 @@ __global__ __launch_bounds__(kPull) void k_pull(fis_sim s, int W,
           }
         }
       }
+      e = en;
+      pre = pre_n;
+      val = val_n;
     }
   }
   __syncthreads();
@@ -624,7 +641,16 @@ __global__ __launch_bounds__(kPull) void k_pull(fis_sim s, int W,
 
 #pragma unroll 1
   for (int k = 0; k < W; ++k) {
-    int32_t I = acc[k * kPull + tid];
+    int32_t* row = acc + k * kPull;
+    int32_t I = row[tid];
+    // the wave's 64 cells of this row are free once read: two of them
+    // collect the wave's spike words (LDS atomics of the spiking lanes)
+    uint64_t* scr = (uint64_t*)(row + (tid & ~(kWave - 1)));
+    if (lane == 0) {
+      scr[0] = 0ull;
+      scr[1] = 0ull;
+    }
+    __builtin_amdgcn_wave_barrier();
     if (k < ring_steps && act) {  // input carried over from a restore
       const int32_t x = rp[(int64_t)rslot * nb];
       if (x != 0) {
@@ -635,17 +661,20 @@ __global__ __launch_bounds__(kPull) void k_pull(fis_sim s, int W,
     const float spk = lif_step(s, c, I, (sb >> k) & 1u);
     const bool fired = act && spk != 0.0f;
     nspk += (fired && t0 + k < t_stop) ? 1u : 0u;
+    const bool sends = fired && transmit;
+    if (sends) atomicOr(&scr[0], 1ull << lane);
+    if (fired && s.raster) atomicOr(&scr[1], 1ull << lane);
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t msk = scr[0];
     // `act` is uniform over a wave (n * bstride is a multiple of 64): the
-    // waves past the last neuron of the last tile have no raster word
-    if (s.raster && act) {
-      const uint64_t all = group_or(fired ? (1ull << lane) : 0ull, kWave);
-      s.raster[((int64_t)rrow * nblk + (wave >> 2)) * FIS_LINE_WORDS +
-               (wave & 3)] = all;
+    // waves past the last neuron of the last tile store nothing
+    if (lane == 0 && act) {
+      wp[wslot * words] = msk;
+      if (s.raster)
+        s.raster[((int64_t)rrow * nblk + (wave >> 2)) * FIS_LINE_WORDS +
+                 (wave & 3)] = scr[1];
     }
     if (s.raster) rrow = (rrow + 1 == s.raster_len) ? 0 : rrow + 1;
-    const bool sends = fired && transmit;
-    const uint64_t msk = group_or(sends ? (1ull << lane) : 0ull, kWave);
-    if (act) wp[wslot * words] = msk;  // every lane: the same word
     abits |= msk != 0 ? (1u << k) : 0u;
     rslot = (rslot + 1 == FIS_RING_LEN) ? 0 : rslot + 1;
     wslot = (wslot + 1 == kRec) ? 0 : wslot + 1;

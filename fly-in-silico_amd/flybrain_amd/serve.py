@@ -1,0 +1,333 @@
+"""Serving: continuous batching of simulation requests on one GPU.
+
+The reference serves one simulation job at a time: the worker takes the
+next job, builds an engine, runs it to completion and writes the result
+(lab/flybrain/worker.py:298-313, :454-505; cancellation and truncation in
+lab/flybrain/jobs.py:348-412).  Here the requests share one resident batch
+of trial slots:
+
+  * capacity is chosen from free HBM: ``bytes_per_trial`` of state, delay
+    ring and spike records per slot (``Server.capacity``);
+  * requests that have arrived are admitted into free slots between two
+    windows; the slot is reset on the device (``fis_reset_trials``), its
+    own clock starts at 0 (Philox stream and rate schedule are the
+    request's own, independent of when and where it runs) and its spikes
+    are counted until its last step (``t_end``);
+  * all admitted requests advance together, one window of 20 steps per
+    launch, and a request whose last step has run is read out (its column
+    of spike counts) and its slot freed for the next arrival.
+
+Trials never interact, so a request's result is bit-for-bit the result of
+the same request run alone; ``tests/test_gpu_serve.py`` checks this.
+"""
+
+from __future__ import annotations
+
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .device import NO_END, DeviceNet, DeviceSim, check_rates_hz, pick_road
+
+
+@dataclass
+class SimRequest:
+    """One simulation: ``steps`` steps of 0.1 ms with Poisson drive into
+    ``drive`` at ``rate_hz``, m_out = 0 on ``silenced``, Philox ``seed``."""
+
+    rid: int
+    steps: int
+    drive: np.ndarray
+    rate_hz: float
+    silenced: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
+    seed: int = 0
+    arrival_s: float = 0.0
+
+
+@dataclass
+class SimResult:
+    rid: int
+    counts: np.ndarray  # int64 [n]: spikes per neuron over the request
+    steps: int
+    arrival_s: float
+    admitted_s: float
+    done_s: float
+    slot: int
+
+    @property
+    def latency_s(self):
+        """Completion latency: result on the host minus arrival."""
+        return self.done_s - self.arrival_s
+
+
+class Scheduler:
+    """Queue and slot bookkeeping (no device work).
+
+    ``submit`` queues requests in arrival order; ``admit`` moves arrived
+    requests into free slots (lowest slot first) at global step ``t``;
+    ``finished`` frees the slots whose request's last step is < ``t``.
+    """
+
+    def __init__(self, slots):
+        if slots < 1:
+            raise ValueError("need at least one slot")
+        self.slots = int(slots)
+        self.free = list(range(self.slots))
+        self.active = {}  # slot -> (request, t_start, admitted_s)
+        self.queue = deque()
+
+    def submit(self, req):
+        """Queue a request; the queue stays in arrival order (ties in
+        submission order)."""
+        if req.steps < 1:
+            raise ValueError("request %r: steps must be >= 1" % (req.rid,))
+        k = len(self.queue)
+        while k > 0 and self.queue[k - 1].arrival_s > req.arrival_s:
+            k -= 1
+        self.queue.insert(k, req)
+
+    def admit(self, now_s, t):
+        out = []
+        while self.free and self.queue and self.queue[0].arrival_s <= now_s:
+            req = self.queue.popleft()
+            slot = min(self.free)
+            self.free.remove(slot)
+            self.active[slot] = (req, int(t), float(now_s))
+            out.append((slot, req))
+        return out
+
+    def finished(self, t):
+        done = [s for s, (req, t0, _) in self.active.items()
+                if t0 + req.steps <= t]
+        out = []
+        for s in sorted(done):
+            req, t0, adm = self.active.pop(s)
+            self.free.append(s)
+            out.append((s, req, t0, adm))
+        return out
+
+    def next_arrival(self):
+        return self.queue[0].arrival_s if self.queue else None
+
+    @property
+    def busy(self):
+        return bool(self.active)
+
+
+def bytes_per_trial(n, road):
+    """HBM held per trial slot: v, g, vhold (12 B), st, fl (2 B), int32
+    ring (80 B), kicks and counts (8 B), events (6 B) and, on the pull
+    road, 40 steps of spike records (5 B) -- per neuron."""
+    per = 12 + 2 + 80 + 8 + 6 + (5 if road == "pull" else 0)
+    return per * int(n)
+
+
+class Server:
+    """A resident batch of ``slots`` trials serving SimRequests."""
+
+    def __init__(self, net: DeviceNet, slots=256, window=20, road="auto",
+                 hbm_fraction=0.5, max_drive=None):
+        self.net = net
+        n = net.n
+        cap = self.capacity(net, road, hbm_fraction)
+        self.slots = int(min(slots, cap))
+        if self.slots < 1:
+            raise MemoryError("no HBM for a single trial slot")
+        self.road = pick_road(self.slots, road)
+        self.window = int(window)
+        if not 1 <= self.window <= L.MAX_WINDOW:
+            raise ValueError("window must be 1..%d" % L.MAX_WINDOW)
+        self.sim = DeviceSim(net, self.slots, counts=True, road=self.road,
+                             seeds=np.zeros(self.slots, np.uint64),
+                             trial_ids=np.zeros(self.slots, np.uint32))
+        # idle slots count nothing
+        self.sim.t_end_host[:] = 0
+        self.sim.t_end.copy_(torch.from_numpy(self.sim.t_end_host))
+        self.max_drive = int(max_drive or n)
+        # drive table of fixed size (one rate per slot), so a captured
+        # graph stays valid when requests come and go
+        self._cap = self.slots * self.max_drive if max_drive else 0
+        self._table = None
+        self._rates = np.zeros((1, self.slots), np.float32)
+        self.sched = Scheduler(self.slots)
+        self._graph = None
+        self._drive = {}  # slot -> neuron indices
+
+    @staticmethod
+    def capacity(net, road="auto", hbm_fraction=0.5):
+        free, _ = torch.cuda.mem_get_info(net.device)
+        per = bytes_per_trial(net.n, "pull" if road != "push" else "push")
+        return int(free * hbm_fraction // per)
+
+    # ------------------------------------------------------------ table
+    def _build_table(self):
+        ent = []
+        for slot, ne in self._drive.items():
+            idx = ne.astype(np.int64) * self.sim.bstride + slot
+            e = np.empty((idx.size, 2), np.uint32)
+            e[:, 0] = idx
+            e[:, 1] = slot
+            ent.append(e)
+        ent = np.concatenate(ent) if ent else np.zeros((0, 2), np.uint32)
+        dev = self.sim.device
+        if self._cap:
+            if ent.shape[0] > self._cap:
+                raise ValueError("more driven neuron-trials than max_drive")
+            pad = np.zeros((self._cap - ent.shape[0], 2), np.uint32)
+            pad[:, 1] = self.slots  # a zero-rate group
+            ent = np.concatenate([ent, pad])
+            rates = np.zeros((1, self.slots + 1), np.float32)
+            rates[0, :self.slots] = self._rates[0]
+        else:
+            rates = self._rates
+        if self._table is None or self._table["n"] != ent.shape[0] or \
+                self._table["g"] != rates.shape[1]:
+            self._table = {
+                "entries": torch.zeros(max(1, ent.size), dtype=torch.int32,
+                                       device=dev),
+                "rate": torch.zeros(rates.size, dtype=torch.float32,
+                                    device=dev),
+                "n": ent.shape[0], "g": rates.shape[1]}
+            self._graph = None
+        if ent.size:
+            self._table["entries"][:ent.size].copy_(
+                torch.from_numpy(ent.view(np.int32).reshape(-1)))
+        self._table["rate"].copy_(torch.from_numpy(rates.reshape(-1)))
+
+    def _window(self, W):
+        s, tb = self.sim, self._table
+        if tb is not None and tb["n"]:
+            L.check(s.lib.fis_drive_table(
+                s._sp, ctypes_ptr(tb["entries"]), tb["n"],
+                ctypes_ptr(tb["rate"]), tb["g"], 1, 1 << 30, W,
+                ctypes_stream()), "fis_drive_table")
+        s.step_launch(W)
+
+    # ------------------------------------------------------------ slots
+    def _admit(self, admits):
+        s = self.sim
+        slots = [slot for slot, _ in admits]
+        out_ne, out_tr, free_ne, free_tr = [], [], [], []
+        for slot, req in admits:
+            drv = np.unique(np.asarray(req.drive, np.int64))
+            check_rates_hz(np.array([req.rate_hz]), "request %r" % (req.rid,))
+            if drv.size > self.max_drive:
+                raise ValueError("request %r drives %d neurons > max_drive"
+                                 % (req.rid, drv.size))
+            sil = np.unique(np.asarray(req.silenced, np.int64))
+            out_ne.append(sil)
+            out_tr.append(np.full(sil.size, slot))
+            free_ne.append(drv)
+            free_tr.append(np.full(drv.size, slot))
+            self._drive[slot] = drv
+            self._rates[0, slot] = np.float32(req.rate_hz)
+        s.restart_trials(slots, [int(r.seed) for _, r in admits],
+                         [s.t + int(r.steps) for _, r in admits],
+                         out_neurons=np.concatenate(out_ne),
+                         out_trials=np.concatenate(out_tr),
+                         free_neurons=np.concatenate(free_ne),
+                         free_trials=np.concatenate(free_tr))
+        self._build_table()
+
+    def _retire(self, slot):
+        self._drive.pop(slot, None)
+        self._rates[0, slot] = 0.0
+        s = self.sim
+        s.t_end_host[slot] = 0
+        s.t_end.copy_(torch.from_numpy(s.t_end_host))
+
+    # ------------------------------------------------------------ loop
+    def serve(self, requests, realtime=True, on_result=None):
+        """Serve ``requests`` (arrival_s relative to the call) and return
+        (results in completion order, stats).  With ``realtime`` the loop
+        admits a request only once its arrival time has passed on the wall
+        clock; otherwise arrival times are ignored (all queued at 0)."""
+        reqs = sorted(requests, key=lambda r: r.arrival_s)
+        for r in reqs:
+            if not realtime:
+                r.arrival_s = 0.0
+            self.sched.submit(r)
+        results = []
+        start = time.perf_counter()
+        busy_windows = 0
+        sync = torch.cuda.synchronize
+        while self.sched.queue or self.sched.busy:
+            now = time.perf_counter() - start
+            admits = self.sched.admit(now, self.sim.t)
+            if admits:
+                self._admit(admits)
+            if not self.sched.busy:
+                nxt = self.sched.next_arrival()
+                if nxt is not None:
+                    time.sleep(max(0.0, min(nxt - now, 0.001)))
+                continue
+            # run until the next completion, at most one window per launch
+            self._window(self.window)
+            busy_windows += 1
+            done = self.sched.finished(self.sim.t)
+            if not done:
+                continue
+            for slot, req, t0, adm in done:
+                counts = self.sim.trial_counts(slot)  # synchronises
+                res = SimResult(req.rid, counts, req.steps, req.arrival_s,
+                                adm, time.perf_counter() - start, slot)
+                results.append(res)
+                self._retire(slot)
+                if on_result is not None:
+                    on_result(res)
+            self._build_table()
+        sync()
+        wall = time.perf_counter() - start
+        return results, summarize(results, wall, self.slots, busy_windows,
+                                  self.window)
+
+
+def summarize(results, wall, slots, windows, window):
+    lat = np.array([r.latency_s for r in results]) if results else np.zeros(1)
+    steps = sum(r.steps for r in results)
+    return {"requests": len(results), "slots": int(slots),
+            "wall_s": round(wall, 4),
+            "trial_s_per_s": round(steps * 1e-4 / wall, 3) if wall else 0.0,
+            "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+            "mean_ms": round(1e3 * float(lat.mean()), 3),
+            "windows": int(windows), "window_steps": int(window)}
+
+
+def poisson_requests(n, count, rate_req_s, seed=0, steps=(500, 20_000),
+                     drive=(20, 200), drive_hz=(50.0, 250.0),
+                     silenced=(0, 50)):
+    """Requests of the serving workload: Poisson arrivals at
+    ``rate_req_s``, duration U(500, 20,000) steps, 20-200 driven neurons
+    at U(50, 250) Hz, 0-50 silenced (m_out = 0) neurons, own seed."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate_req_s, size=count))
+    out = []
+    for k in range(count):
+        nd = int(rng.integers(drive[0], drive[1] + 1))
+        ns = int(rng.integers(silenced[0], silenced[1] + 1))
+        out.append(SimRequest(
+            rid=k, steps=int(rng.integers(steps[0], steps[1] + 1)),
+            drive=rng.choice(n, nd, replace=False),
+            rate_hz=float(rng.uniform(*drive_hz)),
+            silenced=rng.choice(n, ns, replace=False),
+            seed=int(rng.integers(0, 2**63)), arrival_s=float(t[k])))
+    return out
+
+
+def ctypes_ptr(t):
+    import ctypes
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def ctypes_stream():
+    import ctypes
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+__all__ = ["SimRequest", "SimResult", "Scheduler", "Server", "NO_END",
+           "bytes_per_trial", "poisson_requests", "summarize"]

@@ -1,0 +1,75 @@
+"""Continuous batching on the GPU: requests admitted into slots between
+windows, slots reused, every result identical to the same request run
+alone, and one request against the oracle (same Philox stream)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+from flybrain_amd import connectome as C
+from flybrain_amd.device import DeviceNet
+from flybrain_amd.serve import Server, SimRequest
+
+pytestmark = pytest.mark.gpu
+
+
+def requests(n, count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        out.append(SimRequest(
+            rid=k, steps=int(rng.integers(25, 260)),
+            drive=rng.choice(n, int(rng.integers(5, 40)), replace=False),
+            rate_hz=float(rng.uniform(200, 1500)),
+            silenced=rng.choice(n, int(rng.integers(0, 12)), replace=False),
+            seed=int(rng.integers(0, 2**63))))
+    return out
+
+
+@pytest.fixture(scope="module")
+def net():
+    conn = C.synthetic(1500, None, mean_out=60, count_exp=1.15, n_groups=6,
+                       seed=31)
+    return DeviceNet(conn)
+
+
+@pytest.mark.parametrize("road,slots", [("push", 6), ("pull", 6),
+                                        ("pull", 70)])
+def test_result_is_the_same_alone_and_in_the_mix(net, road, slots):
+    reqs = requests(net.n, 24, seed=slots)
+    mix, stats = Server(net, slots=slots, road=road).serve(
+        [SimRequest(**vars(r)) for r in reqs], realtime=False)
+    assert stats["requests"] == len(reqs)
+    got = {r.rid: r for r in mix}
+    assert sorted(got) == [r.rid for r in reqs]
+    assert len({r.slot for r in mix}) == min(slots, len(reqs))  # reused
+    assert any(r.counts.sum() > 0 for r in mix)
+    for r in reqs[:10]:
+        alone, _ = Server(net, slots=1, road="push").serve(
+            [SimRequest(**vars(r))], realtime=False)
+        np.testing.assert_array_equal(got[r.rid].counts, alone[0].counts,
+                                      err_msg="request %d" % r.rid)
+
+
+def test_served_request_matches_the_oracle(net):
+    """A request admitted mid-run (its slot restarted at a later clock)
+    gives the counts of the oracle running it from step 0."""
+    reqs = requests(net.n, 5, seed=99)
+    res, _ = Server(net, slots=2, road="pull").serve(
+        [SimRequest(**vars(r)) for r in reqs], realtime=False)
+    late = max(res, key=lambda r: r.admitted_s)
+    r = reqs[late.rid]
+    conn = net.conn
+    ora = O.OracleEngine(conn, 1, seeds=[r.seed], trial_ids=[0],
+                         no_refractory=np.unique(r.drive))
+    if len(r.silenced):
+        ora.set_masks(out=list(r.silenced))
+    rates = np.zeros((1, conn.n), np.float32)
+    rates[0, r.drive] = r.rate_hz
+    counts = np.zeros(conn.n, np.int64)
+    for _ in range(r.steps):
+        counts += ora.step_rates(rates)[0].astype(np.int64)
+    np.testing.assert_array_equal(late.counts, counts)
+    assert counts.sum() > 0
