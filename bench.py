@@ -95,6 +95,45 @@ def scatter_bytes(conn, raster_rows):
     return rows + 8 * edges_trials
 
 
+def serve_bench(net, conn, args):
+    """Serving workload: Poisson arrivals of simulation requests
+    (500-20,000 steps, 20-200 driven neurons at 50-250 Hz, 0-50 silenced,
+    own seed) served by continuous batching in resident slots, and the same
+    requests run back to back by the plain B = 1 engine.  Both in real
+    time; results must be identical request by request."""
+    import numpy as np
+
+    from flybrain_amd.serve import Server, SimRequest, poisson_requests
+
+    reqs = poisson_requests(conn.n, args.serve_requests, args.serve_rate,
+                            seed=1)
+
+    def copies():
+        return [SimRequest(**vars(r)) for r in reqs]
+
+    warm = [SimRequest(rid=-1 - k, steps=200, drive=reqs[k].drive,
+                       rate_hz=reqs[k].rate_hz, seed=k) for k in range(2)]
+    out = {"workload": "poisson_requests", "rate_req_s": args.serve_rate,
+           "requests": len(reqs),
+           "mean_steps": float(np.mean([r.steps for r in reqs]))}
+    runs = {}
+    for name, kw in (("batched", dict(slots=args.serve_slots)),
+                     ("b1_back_to_back", dict(slots=1, road="push"))):
+        srv = Server(net, max_drive=200, graph_windows=args.graph_windows,
+                     **kw)
+        srv.serve(warm, realtime=False)
+        res, st = srv.serve(copies(), realtime=True)
+        runs[name] = {r.rid: r.counts for r in res}
+        st["road"] = srv.road
+        out[name] = st
+        log("serve %s: %s" % (name, st))
+        del srv
+    a, b = runs["batched"], runs["b1_back_to_back"]
+    out["results_identical"] = bool(
+        a.keys() == b.keys() and all(np.array_equal(a[k], b[k]) for k in a))
+    return out
+
+
 def profiled_traffic(kernel):
     """HBM bytes per launch of ``kernel`` at its largest grid, from the
     rocprofv3 FETCH_SIZE / WRITE_SIZE summaries committed in profiles/
@@ -208,6 +247,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=("all", "b256", "b1"), default="all")
+    ap.add_argument("--serve-requests", type=int, default=64,
+                    help="requests of the serving run (0: skip it)")
+    ap.add_argument("--serve-rate", type=float, default=100.0,
+                    help="Poisson arrival rate of requests per second")
+    ap.add_argument("--serve-slots", type=int, default=256)
     ap.add_argument("--road", choices=("auto", "push", "pull"), default="auto",
                     help="auto: pull road for 64..1024 trials, push below")
     ap.add_argument("--no-graph", action="store_true",
@@ -326,6 +370,11 @@ def main():
         out.setdefault("spikes_per_trial_step", {})[name] = round(
             rr["spikes"] / (rr["wl"].batch * (rr["steps"] + args.warmup)),
             3)
+    if args.serve_requests > 0 and args.only == "all":
+        sv = serve_bench(net, conn, args)
+        out["serving"] = sv
+        out["p50_ms"] = sv["batched"]["p50_ms"]
+        out["p99_ms"] = sv["batched"]["p99_ms"]
     if not args.no_cpu:
         b1 = WL.plain_b1(conn.n, seed=0)
         out["cpu_baseline"] = cpu_baseline(conn, b1, args.cpu_seconds)

@@ -499,9 +499,9 @@ __global__ __launch_bounds__(kBlock) void k_resend(fis_sim s,
 
 // ------------------------------------------------------------- pull road
 //
-// One launch per window.  Workgroup = FIS_PULL_TILE neuron-trials = P
-// neurons x all bstride trials (bstride a power of two, 64..1024), so the
-// tile is a contiguous idx range.  Records are laid out [pre][slot][word]:
+// One launch per window.  Workgroup = max(FIS_PULL_TILE, bstride)
+// neuron-trials = P neurons x all bstride trials (bstride a power of two,
+// 64..1024), so the tile is a contiguous idx range.  Records are laid out [pre][slot][word]:
 // one neuron's 40 steps x bstride bits (320 B at 64 trials) belong to the
 // workgroup that owns the neuron, and the 20 steps of one delay horizon
 // sit in a few lines for the gather.
@@ -519,7 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_resend(fis_sim s,
 // disjoint slots mod 40 for W <= 20, so no workgroup reads what another
 // writes in the same launch.
 
-constexpr int kPull = FIS_PULL_TILE;
+constexpr int kPullTile = FIS_PULL_TILE;
 constexpr int kRec = FIS_REC_SLOTS;
 constexpr uint64_t kMask40 = (1ull << kRec) - 1;
 
@@ -529,16 +529,21 @@ __device__ __forceinline__ uint64_t rot40(uint64_t x, int r) {
   return r == 0 ? x : (((x << r) | (x >> (kRec - r))) & kMask40);
 }
 
-template <int WMAX>
-__global__ __launch_bounds__(kPull) void k_pull(fis_sim s, int W,
-                                                int ring_steps) {
-  // exactly WMAX x 4 KB of LDS and nothing else, so that two workgroups
-  // share a CU's 160 KB
+template <int BS>
+constexpr int pull_tile() {
+  return BS > kPullTile ? BS : kPullTile;
+}
+
+template <int WMAX, int BS>
+__global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
+                                                          int ring_steps) {
+  constexpr int kPull = pull_tile<BS>();  // neuron-trials per workgroup
+  // WMAX x 2 KB of LDS and nothing else: several workgroups per CU
   __shared__ int32_t acc[WMAX * kPull];
 
-  const int bs = s.bstride;
-  const int P = kPull / bs;     // neurons per tile
-  const int words = bs >> 6;    // 64-trial words per neuron
+  constexpr int bs = BS;           // trial stride (== s.bstride)
+  constexpr int P = kPull / BS;    // neurons per tile
+  constexpr int words = BS >> 6;   // 64-trial words per neuron
   const int tid = (int)threadIdx.x;
   const int p0 = (int)blockIdx.x * P;
   const int np = min(P, s.n - p0);
@@ -550,9 +555,9 @@ __global__ __launch_bounds__(kPull) void k_pull(fis_sim s, int W,
 
   for (int j = 0; j < W; ++j) acc[j * kPull + tid] = 0;
   // the tile's in-edge offsets: uniform, held in scalar registers
-  int off[kPull / kWave + 1];
+  int off[P + 1];
 #pragma unroll
-  for (int q = 0; q <= kPull / kWave; ++q)
+  for (int q = 0; q <= P; ++q)
     off[q] = q <= np ? s.in_offsets[p0 + q] : 0x7fffffff;
   __syncthreads();
 
@@ -579,7 +584,7 @@ __global__ __launch_bounds__(kPull) void k_pull(fis_sim s, int W,
       }
       int pl = 0;
 #pragma unroll
-      for (int q = 1; q < kPull / kWave; ++q) pl += e >= off[q] ? 1 : 0;
+      for (int q = 1; q < P; ++q) pl += e >= off[q] ? 1 : 0;
       uint64_t m = rot40(s.act[pre], (kRec - rbase) % kRec) & wmask;
       int32_t* ap = acc + pl * bs + (w << 6);
       const uint64_t* rp = s.rec + (int64_t)pre * pstride + w;
@@ -867,7 +872,7 @@ int check_sim(const fis_sim* s) {
 int check_pull(const fis_sim* s) {
   if (int rc = check_sim(s)) return rc;
   const int bs = s->bstride;
-  if (bs < kWave || bs > kPull || (bs & (bs - 1)) != 0)
+  if (bs < kWave || bs > 1024 || (bs & (bs - 1)) != 0)
     return (int)hipErrorInvalidValue;
   if (!s->in_offsets || !s->in_cols || !s->in_vals || !s->rec || !s->act)
     return (int)hipErrorInvalidValue;
@@ -1026,9 +1031,19 @@ int fis_pull_window(const fis_sim* s, int W, int ring_steps, void* stream) {
   if (W < 1 || W > FIS_MAX_WINDOW || ring_steps < 0)
     return (int)hipErrorInvalidValue;
   const int64_t nb = (int64_t)s->n * s->bstride;
-  const unsigned blocks = (unsigned)((nb + kPull - 1) / kPull);
-  hipLaunchKernelGGL(k_pull<FIS_MAX_WINDOW>, dim3(blocks), dim3(kPull), 0,
-                     (hipStream_t)stream, *s, W, ring_steps);
+  auto kern = k_pull<FIS_MAX_WINDOW, 256>;
+  int tile = 0;
+  switch (s->bstride) {
+    case 64: kern = k_pull<FIS_MAX_WINDOW, 64>; tile = pull_tile<64>(); break;
+    case 128: kern = k_pull<FIS_MAX_WINDOW, 128>; tile = pull_tile<128>(); break;
+    case 256: kern = k_pull<FIS_MAX_WINDOW, 256>; tile = pull_tile<256>(); break;
+    case 512: kern = k_pull<FIS_MAX_WINDOW, 512>; tile = pull_tile<512>(); break;
+    case 1024: kern = k_pull<FIS_MAX_WINDOW, 1024>; tile = pull_tile<1024>(); break;
+    default: return (int)hipErrorInvalidValue;
+  }
+  const unsigned blocks = (unsigned)((nb + tile - 1) / tile);
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(tile), 0, (hipStream_t)stream,
+                     *s, W, ring_steps);
   hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, (hipStream_t)stream,
                      s->clock, W);
   return (int)hipGetLastError();

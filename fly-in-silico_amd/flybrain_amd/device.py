@@ -31,12 +31,12 @@ PULL_MIN_BATCH = 64
 def pick_road(batch: int, road: str = "auto") -> str:
     """``auto``: the pull road from 64 trials up (to 1,024), else push."""
     if road == "auto":
-        return "pull" if PULL_MIN_BATCH <= batch <= L.PULL_TILE else "push"
+        return "pull" if PULL_MIN_BATCH <= batch <= L.PULL_MAX else "push"
     if road not in ROADS:
         raise ValueError("road %r not in %s" % (road, ROADS))
-    if road == "pull" and batch > L.PULL_TILE:
+    if road == "pull" and batch > L.PULL_MAX:
         raise ValueError("the pull road holds at most %d trials"
-                         % L.PULL_TILE)
+                         % L.PULL_MAX)
     return road
 
 

@@ -130,7 +130,7 @@ class Server:
     """A resident batch of ``slots`` trials serving SimRequests."""
 
     def __init__(self, net: DeviceNet, slots=256, window=20, road="auto",
-                 hbm_fraction=0.5, max_drive=None):
+                 hbm_fraction=0.5, max_drive=None, graph_windows=0):
         self.net = net
         n = net.n
         cap = self.capacity(net, road, hbm_fraction)
@@ -155,6 +155,10 @@ class Server:
         self._rates = np.zeros((1, self.slots), np.float32)
         self.sched = Scheduler(self.slots)
         self._graph = None
+        self.graph_windows = int(graph_windows)
+        if self.graph_windows and not max_drive:
+            raise ValueError("graph replay needs a fixed-size drive table "
+                             "(max_drive)")
         self._drive = {}  # slot -> neuron indices
 
     @staticmethod
@@ -206,6 +210,37 @@ class Server:
                 ctypes_ptr(tb["rate"]), tb["g"], 1, 1 << 30, W,
                 ctypes_stream()), "fis_drive_table")
         s.step_launch(W)
+
+    def _capture(self):
+        """One HIP graph of ``graph_windows`` windows.  The drive table,
+        the per-trial clocks and the step counter are read from device
+        memory, so the graph stays valid while requests come and go."""
+        g = torch.cuda.CUDAGraph()
+        t0 = self.sim.t  # capture records launches; the host clock stays
+        try:
+            with torch.cuda.graph(g):
+                for _ in range(self.graph_windows):
+                    self._window(self.window)
+        finally:
+            self.sim.t = t0
+        self._graph = g
+
+    def _advance(self):
+        """Run windows up to the next completion: a whole graph when no
+        slot is free (nothing could be admitted meanwhile) and no request
+        ends inside it, else one window."""
+        per = self.graph_windows * self.window
+        if per and not self.sched.free:
+            left = min(t0 + req.steps for req, t0, _ in
+                       self.sched.active.values()) - self.sim.t
+            if left >= per:
+                if self._graph is None:
+                    self._capture()
+                self._graph.replay()
+                self.sim.t += per
+                return self.graph_windows
+        self._window(self.window)
+        return 1
 
     # ------------------------------------------------------------ slots
     def _admit(self, admits):
@@ -265,9 +300,7 @@ class Server:
                 if nxt is not None:
                     time.sleep(max(0.0, min(nxt - now, 0.001)))
                 continue
-            # run until the next completion, at most one window per launch
-            self._window(self.window)
-            busy_windows += 1
+            busy_windows += self._advance()
             done = self.sched.finished(self.sim.t)
             if not done:
                 continue

@@ -48,7 +48,8 @@ extern "C" {
 #define FIS_LINE_INTS 32    /* 128 bytes = 32 x 4-byte ints               */
 #define FIS_EVENT_ALIGN 8   /* 8 events = 128 bytes                       */
 #define FIS_REC_SLOTS 40    /* pull road: spike records of steps t mod 40 */
-#define FIS_PULL_TILE 1024  /* pull road: neuron-trials per workgroup     */
+#define FIS_PULL_TILE 512   /* pull road: neuron-trials per workgroup
+                               (bstride 1024: one neuron, 1024)           */
 
 /* flag bits of `fl`, one byte per neuron-trial (sim.py:393-410, :866-900) */
 #define FIS_FIRE_OFF 0x01u  /* m_fire == 0: spike output suppressed        */
@@ -213,8 +214,8 @@ int fis_resend(const fis_sim* s, const int32_t* idx, int n_idx, int slot,
                void* stream);
 
 /* ---- pull road: one fused launch per window (bstride a power of two,
- * 64 .. 1024).  Each workgroup owns 1024 neuron-trials (1024 / bstride
- * neurons x all trials).  It sums the input of its neurons for the W steps
+ * 64 .. 1024).  Each workgroup owns max(512, bstride) neuron-trials
+ * (that many / bstride neurons x all trials).  It sums the input of its neurons for the W steps
  * t0 .. t0+W-1 from the spike records of steps t0-20 .. t0+W-21 over their
  * in-edges into LDS (exact int32), then runs the W update steps of
  * Engine._core (sim.py:920-953) from LDS and writes the window's own

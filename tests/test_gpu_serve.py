@@ -73,3 +73,20 @@ def test_served_request_matches_the_oracle(net):
         counts += ora.step_rates(rates)[0].astype(np.int64)
     np.testing.assert_array_equal(late.counts, counts)
     assert counts.sum() > 0
+
+
+@pytest.mark.parametrize("road,slots", [("push", 1), ("pull", 3)])
+def test_graph_replay_serves_the_same(net, road, slots):
+    """Whole graphs replayed while no slot is free give the results of
+    window-by-window launches."""
+    reqs = requests(net.n, 8, seed=5)
+    eager, _ = Server(net, slots=slots, road=road).serve(
+        [SimRequest(**vars(r)) for r in reqs], realtime=False)
+    srv = Server(net, slots=slots, road=road, max_drive=64, graph_windows=2)
+    graph, st = srv.serve([SimRequest(**vars(r)) for r in reqs],
+                          realtime=False)
+    assert srv._graph is not None, "the graph road was not taken"
+    a = {r.rid: r.counts for r in eager}
+    for r in graph:
+        np.testing.assert_array_equal(r.counts, a[r.rid],
+                                      err_msg="request %d" % r.rid)
