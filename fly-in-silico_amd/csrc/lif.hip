@@ -536,7 +536,9 @@ constexpr int pull_tile() {
 
 template <int WMAX, int BS>
 __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
-                                                          int ring_steps) {
+                                                          int ring_steps,
+                                                          int n_tiles,
+                                                          int per_xcd) {
   constexpr int kPull = pull_tile<BS>();  // neuron-trials per workgroup
   // WMAX x 2 KB of LDS and nothing else: several workgroups per CU
   __shared__ int32_t acc[WMAX * kPull];
@@ -545,7 +547,13 @@ __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
   constexpr int P = kPull / BS;    // neurons per tile
   constexpr int words = BS >> 6;   // 64-trial words per neuron
   const int tid = (int)threadIdx.x;
-  const int p0 = (int)blockIdx.x * P;
+  // XCD-aware tile order: workgroup b is dispatched to XCD b mod 8, and
+  // XCD x takes the consecutive tiles [x * per_xcd, (x + 1) * per_xcd) of
+  // the neuron order, so one XCD's L2 holds the spike records its
+  // neurons' presynaptic partners share
+  const int tile = (int)(blockIdx.x & 7u) * per_xcd + (int)(blockIdx.x >> 3);
+  if (tile >= n_tiles) return;  // whole workgroup, before any barrier
+  const int p0 = tile * P;      // first position of the tile
   const int np = min(P, s.n - p0);
   const int64_t t0 = *s.clock;
   const int rbase = (int)((t0 + kRec - 20) % kRec);  // slot of step t0-20
@@ -566,6 +574,8 @@ __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
     constexpr int kIn = 8;
     const int w = tid & (words - 1);
     const int G = kPull / words;
+    const bool wlive = !s.word_end || t0 < s.word_end[w];
+    const bool nowrap = rbase + W <= kRec;  // uniform
     const int e1 = s.in_offsets[p0 + np];
     int e = off[0] + tid / words;
     int pre = 0;
@@ -585,9 +595,36 @@ __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
       int pl = 0;
 #pragma unroll
       for (int q = 1; q < P; ++q) pl += e >= off[q] ? 1 : 0;
-      uint64_t m = rot40(s.act[pre], (kRec - rbase) % kRec) & wmask;
+      uint64_t m =
+          wlive ? rot40(s.act[pre], (kRec - rbase) % kRec) & wmask : 0ull;
       int32_t* ap = acc + pl * bs + (w << 6);
       const uint64_t* rp = s.rec + (int64_t)pre * pstride + w;
+      if (nowrap) {
+        // the horizon's slots rbase .. rbase+W-1 are consecutive: step j's
+        // word sits at a fixed offset from one base (no per-step slot and
+        // address arithmetic); two rounds of 10 predicated loads
+        const uint64_t* rq = rp + rbase * words;
+        const uint32_t m32 = (uint32_t)m;
+#pragma unroll
+        for (int h = 0; h < WMAX; h += 10) {
+          uint64_t x[10];
+#pragma unroll
+          for (int q = 0; q < 10; ++q)
+            x[q] = (h + q < WMAX && ((m32 >> (h + q)) & 1u))
+                       ? rq[(h + q) * words] : 0ull;
+#pragma unroll
+          for (int q = 0; q < 10; ++q) {
+            uint64_t bits = x[q];
+            int32_t* aj = ap + (h + q) * kPull;
+            while (bits) {
+              const int b = __ffsll((long long)bits) - 1;
+              bits &= bits - 1ull;
+              atomicAdd(aj + b, val);
+            }
+          }
+        }
+        m = 0;
+      }
       while (m) {
         int j[kIn];
         uint64_t x[kIn];
@@ -621,12 +658,14 @@ __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
 
   // 2. update
   const int nb = s.n * bs;
-  const int idx = (int)blockIdx.x * kPull + tid;
   const int lane = tid & (kWave - 1);
+  const int pl = tid / bs;           // position within the tile
+  const int tb = tid - pl * bs;      // trial of this lane
+  // uniform over a wave: a neuron of the tile, in a live 64-trial word
+  const bool act = pl < np && (!s.word_end || t0 < s.word_end[tb >> 6]);
+  const int pre = pl < np ? s.order[p0 + pl] : 0;  // neuron of this lane
+  const int idx = pre * bs + tb;
   const int wave = idx >> 6;
-  const bool act = idx < nb;
-  const int pre = idx / bs;
-  const int tb = idx - pre * bs;
   const int nblk = (nb + kBlock - 1) / kBlock;  // raster rows: 256-lane groups
 
   float v = 0.0f, g = 0.0f, vh = 0.0f;
@@ -634,7 +673,10 @@ __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
   if (act) load_cell(s, idx, v, g, st, fl, sb, vh);
   Cell c = make_cell(s, fl, vh, st, v, g);
   const bool transmit = act && !(fl & FIS_OUT_OFF);
-  const int64_t t_stop = act ? end_of(s, tb) : 0;
+  // steps of this window whose spikes still count for the trial
+  const int64_t left = act ? end_of(s, tb) - t0 : 0;
+  const int kstop = left <= 0 ? 0 : (left >= W ? W : (int)left);
+  const bool use_ring = ring_steps > 0;  // uniform
 
   uint64_t* wp = s.rec + (int64_t)pre * pstride + (tb >> 6);
   int32_t* rp = s.ring + idx;
@@ -656,8 +698,8 @@ __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
       scr[1] = 0ull;
     }
     __builtin_amdgcn_wave_barrier();
-    if (k < ring_steps && act) {  // input carried over from a restore
-      const int32_t x = rp[(int64_t)rslot * nb];
+    if (use_ring && k < ring_steps && act) {  // input carried over from a
+      const int32_t x = rp[(int64_t)rslot * nb];  // restore
       if (x != 0) {
         I += x;
         rp[(int64_t)rslot * nb] = 0;
@@ -665,7 +707,7 @@ __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
     }
     const float spk = lif_step(s, c, I, (sb >> k) & 1u);
     const bool fired = act && spk != 0.0f;
-    nspk += (fired && t0 + k < t_stop) ? 1u : 0u;
+    nspk += (fired && k < kstop) ? 1u : 0u;
     const bool sends = fired && transmit;
     if (sends) atomicOr(&scr[0], 1ull << lane);
     if (fired && s.raster) atomicOr(&scr[1], 1ull << lane);
@@ -680,7 +722,10 @@ __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
                  (wave & 3)] = scr[1];
     }
     if (s.raster) rrow = (rrow + 1 == s.raster_len) ? 0 : rrow + 1;
-    abits |= msk != 0 ? (1u << k) : 0u;
+    // msk is the same in every lane: test it in scalar registers
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)msk);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(msk >> 32));
+    abits |= (lo | hi) != 0u ? (1u << k) : 0u;
     rslot = (rslot + 1 == FIS_RING_LEN) ? 0 : rslot + 1;
     wslot = (wslot + 1 == kRec) ? 0 : wslot + 1;
   }
@@ -694,7 +739,7 @@ __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
   if (tid < np) {
     uint32_t bits32 = 0;
     for (int q = 0; q < words; ++q) bits32 |= (uint32_t)acc[tid * bs + q * kWave];
-    uint64_t* a = s.act + p0 + tid;
+    uint64_t* a = s.act + s.order[p0 + tid];
     __hip_atomic_fetch_and(a, ~rot40(wmask, wbase), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     const uint64_t bits = rot40((uint64_t)bits32, wbase);
@@ -874,7 +919,8 @@ int check_pull(const fis_sim* s) {
   const int bs = s->bstride;
   if (bs < kWave || bs > 1024 || (bs & (bs - 1)) != 0)
     return (int)hipErrorInvalidValue;
-  if (!s->in_offsets || !s->in_cols || !s->in_vals || !s->rec || !s->act)
+  if (!s->in_offsets || !s->in_cols || !s->in_vals || !s->rec || !s->act ||
+      !s->order)
     return (int)hipErrorInvalidValue;
   return 0;
 }
@@ -1041,9 +1087,12 @@ int fis_pull_window(const fis_sim* s, int W, int ring_steps, void* stream) {
     case 1024: kern = k_pull<FIS_MAX_WINDOW, 1024>; tile = pull_tile<1024>(); break;
     default: return (int)hipErrorInvalidValue;
   }
-  const unsigned blocks = (unsigned)((nb + tile - 1) / tile);
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(tile), 0, (hipStream_t)stream,
-                     *s, W, ring_steps);
+  const int P = tile / s->bstride;
+  const int n_tiles = (s->n + P - 1) / P;
+  const int per_xcd = (n_tiles + 7) / 8;
+  (void)nb;
+  hipLaunchKernelGGL(kern, dim3((unsigned)(8 * per_xcd)), dim3(tile), 0,
+                     (hipStream_t)stream, *s, W, ring_steps, n_tiles, per_xcd);
   hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, (hipStream_t)stream,
                      s->clock, W);
   return (int)hipGetLastError();

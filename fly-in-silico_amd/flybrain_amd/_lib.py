@@ -20,7 +20,7 @@ CSRC = PKG.parent / "csrc"
 SOURCES = [CSRC / "lif.hip"]
 HEADER = ROOT / "include" / "fis.h"
 LIB_PATH = PKG / "libfis.so"
-ABI_VERSION = 2
+ABI_VERSION = 4
 
 MAX_WINDOW = 20
 RING_LEN = 20
@@ -83,8 +83,8 @@ class FisSim(ctypes.Structure):
         ("spikes_out", P), ("counts", P), ("pop_counts", P),
         ("raster", P), ("raster_len", ctypes.c_int32),
         ("in_offsets", P), ("in_cols", P), ("in_vals", P),
-        ("rec", P), ("act", P),
-        ("t_origin", P), ("t_end", P),
+        ("rec", P), ("act", P), ("order", P),
+        ("t_origin", P), ("t_end", P), ("word_end", P),
     ]
 
 

@@ -79,29 +79,58 @@ class DeviceNet:
         self.offsets = torch.from_numpy(conn.offsets).to(self.device)
         self.cols = torch.from_numpy(conn.cols).to(self.device)
         self.vals = torch.from_numpy(conn.vals).to(self.device)
-        self._in = None
+        self._layouts = {}
 
-    def in_csr(self):
-        """The same synapses by postsynaptic neuron (W[post, pre] CSR,
-        int32 offsets), built once on first use by the pull road."""
-        if self._in is None:
-            c = self.conn
-            if c.n_edges >= (1 << 31):
-                raise ValueError("the pull road indexes edges with int32")
-            pre = np.repeat(np.arange(c.n, dtype=np.int32),
-                            np.diff(c.offsets))
-            order = np.argsort(c.cols, kind="stable")
-            off = np.zeros(c.n + 1, np.int64)
-            np.cumsum(np.bincount(c.cols, minlength=c.n), out=off[1:])
-            dev = self.device
-            self._in = (torch.from_numpy(off.astype(np.int32)).to(dev),
-                        torch.from_numpy(pre[order]).to(dev),
-                        torch.from_numpy(c.vals[order]).to(dev))
-        return self._in
+    locality = True  # pull road: order neurons by group (see pull_layout)
 
-    @property
-    def n_edges(self):
-        return self.conn.n_edges
+    def pull_layout(self, bstride):
+        """The pull road's neuron order and its in-edge CSR by position.
+
+        Units of max(1, 256 / bstride) aligned neurons (what one 256-lane
+        group of the kernel covers, so that every line a workgroup writes
+        stays inside one tile) are sorted by the neuropil group of their
+        first neuron (stable; a partial last unit stays last).  Neurons of
+        one group share most presynaptic partners (80 % of edges stay
+        inside a group), so a run of consecutive tiles -- what one XCD
+        takes -- reads the spike records of few presynaptic neurons.
+        Without groups the order is the identity.  Built once per unit
+        size; int32 offsets.
+        """
+        q = max(1, 256 // int(bstride))
+        key = (q, bool(self.locality))
+        got = self._layouts.get(key)
+        if got is not None:
+            return got
+        c = self.conn
+        if c.n_edges >= (1 << 31):
+            raise ValueError("the pull road indexes edges with int32")
+        n = c.n
+        if c.groups is not None and self.locality:
+            full = n // q
+            unit_group = np.asarray(c.groups, np.int64)[:full * q:q]
+            units = np.argsort(unit_group, kind="stable")
+            order = (units[:, None] * q + np.arange(q)[None, :]).reshape(-1)
+            order = np.concatenate([order, np.arange(full * q, n)])
+        else:
+            order = np.arange(n)
+        # in-edges of neuron order[k] at positions in_off[k] .. in_off[k+1]
+        pre = np.repeat(np.arange(n, dtype=np.int32), np.diff(c.offsets))
+        by_post = np.argsort(c.cols, kind="stable")
+        indeg = np.bincount(c.cols, minlength=n)
+        nat = np.zeros(n + 1, np.int64)
+        np.cumsum(indeg, out=nat[1:])
+        lens = indeg[order]
+        off = np.zeros(n + 1, np.int64)
+        np.cumsum(lens, out=off[1:])
+        take = by_post[np.repeat(nat[order] - off[:-1], lens)
+                       + np.arange(c.n_edges)]
+        dev = self.device
+        got = (torch.from_numpy(order.astype(np.int32)).to(dev),
+               torch.from_numpy(off.astype(np.int32)).to(dev),
+               torch.from_numpy(pre[take]).to(dev),
+               torch.from_numpy(c.vals[take]).to(dev))
+        self._layouts[key] = got
+        return got
 
 
 class DeviceSim:
@@ -159,6 +188,9 @@ class DeviceSim:
         self.t_end_host = np.full(bs, NO_END, np.int64)
         self.t_origin = torch.from_numpy(self.t_origin_host).to(dev)
         self.t_end = torch.from_numpy(self.t_end_host).to(dev)
+        # pull road: the last end step of each 64-trial word (idle words
+        # are skipped); None = every word live
+        self.word_end = None
         self.spikes_out = (torch.zeros(nb, dtype=torch.float32, **z)
                            if spikes_out else None)
         self.counts = torch.zeros(nb, dtype=torch.int32, **z) if counts else None
@@ -171,12 +203,13 @@ class DeviceSim:
                                    * L.LINE_WORDS, dtype=torch.int64, **z)
                        if raster_len else None)
         if self.road == "pull":
-            self.in_offsets, self.in_cols, self.in_vals = net.in_csr()
+            self.order, self.in_offsets, self.in_cols, self.in_vals = \
+                net.pull_layout(bs)
             self.rec = torch.zeros(L.REC_SLOTS * n * (bs // 64),
                                    dtype=torch.int64, **z)
             self.act = torch.zeros(n, dtype=torch.int64, **z)
         else:
-            self.in_offsets = self.in_cols = self.in_vals = None
+            self.order = self.in_offsets = self.in_cols = self.in_vals = None
             self.rec = self.act = None
         self.t = 0  # host mirror of the device clock
         self._open_window = 0
@@ -221,7 +254,8 @@ class DeviceSim:
             t = getattr(self, name)
             setattr(s, name, t.data_ptr() if t is not None else None)
         s.raster_len = self.raster_len
-        for name in ("in_offsets", "in_cols", "in_vals", "rec", "act"):
+        for name in ("in_offsets", "in_cols", "in_vals", "rec", "act",
+                     "order", "word_end"):
             t = getattr(self, name)
             setattr(s, name, t.data_ptr() if t is not None else None)
         self.struct = s
@@ -357,7 +391,32 @@ class DeviceSim:
         self.t_end_host[tr] = te
         self.t_origin.copy_(torch.from_numpy(self.t_origin_host))
         self.t_end.copy_(torch.from_numpy(self.t_end_host))
+        self._push_word_end()
         self._slot_keep = keep  # index buffers live until the launches ran
+
+    def set_trial_end(self, trials, t_end):
+        """Stop counting (and, on the pull road, let a wholly idle 64-trial
+        word be skipped) from global step ``t_end`` on."""
+        tr = self._trials(trials)
+        self.t_end_host[tr] = np.asarray(t_end, np.int64)
+        self.t_end.copy_(torch.from_numpy(self.t_end_host))
+        self._push_word_end()
+
+    def track_idle_words(self):
+        """Pull road: skip 64-trial words whose trials have all ended
+        (t_end); for slot-based serving, where idle slots hold no result."""
+        if self.road != "pull":
+            return
+        if self.word_end is None:
+            self.word_end = torch.zeros(self.bstride // 64, dtype=torch.int64,
+                                        device=self.device)
+            self._sync_struct()
+        self._push_word_end()
+
+    def _push_word_end(self):
+        if self.word_end is not None:
+            we = self.t_end_host.reshape(-1, 64).max(axis=1)
+            self.word_end.copy_(torch.from_numpy(np.ascontiguousarray(we)))
 
     def trial_counts(self, trial):
         """Spike counts of one trial, int64 [n] (one column, on the host)."""

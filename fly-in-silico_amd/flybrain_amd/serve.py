@@ -144,9 +144,10 @@ class Server:
         self.sim = DeviceSim(net, self.slots, counts=True, road=self.road,
                              seeds=np.zeros(self.slots, np.uint64),
                              trial_ids=np.zeros(self.slots, np.uint32))
-        # idle slots count nothing
-        self.sim.t_end_host[:] = 0
-        self.sim.t_end.copy_(torch.from_numpy(self.sim.t_end_host))
+        # idle slots count nothing, and 64-slot words with no live request
+        # are skipped by the pull road's windows
+        self.sim.track_idle_words()
+        self.sim.set_trial_end(np.arange(self.slots), np.zeros(self.slots))
         self.max_drive = int(max_drive or n)
         # drive table of fixed size (one rate per slot), so a captured
         # graph stays valid when requests come and go
@@ -271,9 +272,7 @@ class Server:
     def _retire(self, slot):
         self._drive.pop(slot, None)
         self._rates[0, slot] = 0.0
-        s = self.sim
-        s.t_end_host[slot] = 0
-        s.t_end.copy_(torch.from_numpy(s.t_end_host))
+        self.sim.set_trial_end([slot], [0])
 
     # ------------------------------------------------------------ loop
     def serve(self, requests, realtime=True, on_result=None):

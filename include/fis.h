@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define FIS_ABI_VERSION 2
+#define FIS_ABI_VERSION 4
 #define FIS_MAX_WINDOW 20   /* steps per window: delay horizon 19 + 1     */
 #define FIS_RING_LEN 20     /* int32 ring slots: rec(t-19) .. rec(t)       */
 
@@ -130,11 +130,17 @@ typedef struct {
    * spike records of the last 40 steps.  rec word
    * [pre][t mod 40][w] bit j = trial 64 w + j of `pre` transmitted a spike
    * at step t; act[pre] bit (t mod 40) = some trial of `pre` did.         */
-  const int32_t* in_offsets; /* [n+1]                                  */
+  const int32_t* in_offsets; /* [n+1], by position (see `order`)       */
   const int32_t* in_cols;    /* [E] presynaptic neuron                 */
   const int16_t* in_vals;    /* [E] signed synapse count               */
   uint64_t* rec;             /* [n][40][bstride / 64]                  */
   uint64_t* act;             /* [n]                                    */
+  /* the order in which workgroups take the neurons: position k holds
+   * neuron order[k] (a permutation of 0..n-1 that moves aligned units of
+   * max(1, 256 / bstride) neurons, e.g. sorted by neuropil group, so that
+   * neurons sharing presynaptic partners are updated by the same XCD);
+   * in_offsets / in_cols / in_vals list the in-edges by position.       */
+  const int32_t* order;      /* [n]                                    */
   /* per-trial clock (nullable = 0 and never): trial b's own step is
    * clock - t_origin[b] -- the Philox counter and the drive table's rate
    * segment use it, so a trial's drive does not depend on when its slot
@@ -142,6 +148,13 @@ typedef struct {
    * < t_end[b].  [bstride] each. */
   const int64_t* t_origin;
   const int64_t* t_end;
+  /* pull road (nullable = every word live): [bstride / 64] the largest
+   * t_end of each 64-trial word.  A word whose word_end is <= the first
+   * step of a window holds only finished or never-started trials and is
+   * skipped by that window: no input gathered, no state updated, no
+   * records written.  Its stale records can only reach trials of the same
+   * word, which fis_reset_trials clears when a slot of it restarts. */
+  const int64_t* word_end;
 } fis_sim;
 
 /* Drive table entry: neuron-trial `idx` is driven at rate

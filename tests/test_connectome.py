@@ -130,3 +130,36 @@ def test_save_load(tmp_path):
     d = C.load(tmp_path / "c.npz")
     assert d.fingerprint() == c.fingerprint()
     np.testing.assert_array_equal(d.groups, c.groups)
+
+
+@pytest.mark.parametrize("bstride", [64, 128, 256, 1024])
+@pytest.mark.parametrize("locality", [True, False])
+def test_pull_layout_lists_every_in_edge_by_position(bstride, locality):
+    """The pull road's neuron order is a permutation of aligned units of
+    max(1, 256 / bstride) neurons (grouped by neuropil when asked), and the
+    in-edge CSR by position holds exactly each neuron's row of W."""
+    import torch
+
+    from flybrain_amd.device import DeviceNet
+
+    conn = C.synthetic(1003, None, mean_out=30, n_groups=5, seed=4)
+    net = DeviceNet(conn, "cpu")
+    net.locality = locality
+    order, off, cols, vals = [t.numpy() for t in net.pull_layout(bstride)]
+    assert net.pull_layout(bstride)[0] is not None  # cached
+    q = max(1, 256 // bstride)
+    assert np.array_equal(np.sort(order), np.arange(conn.n))
+    full = conn.n // q
+    units = order[:full * q].reshape(-1, q)
+    assert (units[:, 0] % q == 0).all() and (np.diff(units, axis=1) == 1).all()
+    g = conn.groups[order[:full * q:q]]
+    if locality:
+        assert (np.diff(g.astype(np.int64)) >= 0).all()
+    else:
+        assert np.array_equal(order, np.arange(conn.n))
+    dense = conn.dense_counts()
+    for k in range(0, conn.n, 17):
+        got = np.zeros(conn.n, np.int64)
+        np.add.at(got, cols[off[k]:off[k + 1]], vals[off[k]:off[k + 1]])
+        assert np.array_equal(got, dense[order[k]]), k
+    assert torch.is_tensor(net.pull_layout(bstride)[1])

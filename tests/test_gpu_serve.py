@@ -35,10 +35,14 @@ def net():
     return DeviceNet(conn)
 
 
-@pytest.mark.parametrize("road,slots", [("push", 6), ("pull", 6),
-                                        ("pull", 70)])
-def test_result_is_the_same_alone_and_in_the_mix(net, road, slots):
-    reqs = requests(net.n, 24, seed=slots)
+@pytest.mark.parametrize("road,slots,count", [("push", 6, 24),
+                                              ("pull", 6, 24),
+                                              ("pull", 70, 24),
+                                              ("pull", 128, 150)])
+def test_result_is_the_same_alone_and_in_the_mix(net, road, slots, count):
+    """More requests than slots: slots (and on the pull road whole 64-slot
+    words, skipped while idle) are freed and reused mid-run."""
+    reqs = requests(net.n, count, seed=slots)
     mix, stats = Server(net, slots=slots, road=road).serve(
         [SimRequest(**vars(r)) for r in reqs], realtime=False)
     assert stats["requests"] == len(reqs)
@@ -46,7 +50,7 @@ def test_result_is_the_same_alone_and_in_the_mix(net, road, slots):
     assert sorted(got) == [r.rid for r in reqs]
     assert len({r.slot for r in mix}) == min(slots, len(reqs))  # reused
     assert any(r.counts.sum() > 0 for r in mix)
-    for r in reqs[:10]:
+    for r in reqs[:6] + reqs[-4:]:
         alone, _ = Server(net, slots=1, road="push").serve(
             [SimRequest(**vars(r))], realtime=False)
         np.testing.assert_array_equal(got[r.rid].counts, alone[0].counts,
