@@ -340,6 +340,7 @@ def main():
     achieved = alg / (kt[dom] * 1e-3) / 1e9
     kname = {"update": "k_update", "pull": "k_pull"}.get(dom) or (
         "k_scatter_steps" if head["runner"].sim.bstride >= 64 else "k_scatter")
+    traffic = profiled_traffic(kname)
     out.update({
         "value": round(head["tps"], 3),
         "steps": head["steps"], "warmup": args.warmup,
@@ -352,7 +353,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic(kname),
+                     "traffic": traffic,
+                     "traffic_gbs": (round(traffic / (kt[dom] * 1e-3) / 1e9, 1)
+                                     if traffic else None),
                      "algorithmic_bytes_per_launch": int(alg),
                      "avg_launch_ms": round(kt[dom], 5)},
         "kernels_ms": {k: round(v, 5) for k, v in kt.items()},
