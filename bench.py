@@ -247,9 +247,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=("all", "b256", "b1"), default="all")
-    ap.add_argument("--serve-requests", type=int, default=64,
+    ap.add_argument("--serve-requests", type=int, default=128,
                     help="requests of the serving run (0: skip it)")
-    ap.add_argument("--serve-rate", type=float, default=100.0,
+    ap.add_argument("--serve-rate", type=float, default=200.0,
                     help="Poisson arrival rate of requests per second")
     ap.add_argument("--serve-slots", type=int, default=256)
     ap.add_argument("--road", choices=("auto", "push", "pull"), default="auto",

@@ -192,13 +192,15 @@ __device__ __forceinline__ int prefix_count(bool p, int lane, int& total) {
 
 // Spike compaction of one step: the raster row (optional) and the events
 // of transmitted spikes, appended to this wave's segment.  `pre` and `tb`
-// are the lane's neuron and trial (computed once per thread).  Stores are
+// are the lane's neuron and trial (computed once per thread); `live` is
+// whether the wave holds neuron-trials at all (uniform over the wave when
+// bstride >= 64, the only case that reads it).  Stores are
 // made by every lane (identical values, or a sink slot for lanes with
 // nothing to write), so no branch narrows the wave around them.
-__device__ __forceinline__ void emit(const fis_sim& s, bool fired, bool sends,
-                                     int lane, int pre, int tb, int wave,
-                                     int nblk, int rrow, int slot, int k,
-                                     fis_event* evs, int& nev) {
+__device__ __forceinline__ void emit(const fis_sim& s, bool live, bool fired,
+                                     bool sends, int lane, int pre, int tb,
+                                     int wave, int nblk, int rrow, int slot,
+                                     int k, fis_event* evs, int& nev) {
   const int bs = s.bstride;
   if (s.raster) {
     const uint64_t all = group_or(fired ? (1ull << lane) : 0ull, kWave);
@@ -215,7 +217,9 @@ __device__ __forceinline__ void emit(const fis_sim& s, bool fired, bool sends,
     e.slot = (uint16_t)slot;
     e.tb = (uint16_t)(tb - lane);  // trial of lane 0
     e.mask = any ? group_or(sends ? (1ull << lane) : 0ull, kWave) : 0ull;
-    evs[k] = e;  // every lane writes the same record
+    // every lane writes the same record; a wave past the last neuron of the
+    // last workgroup has no segment (the buffer holds n_waves segments)
+    if (live) evs[k] = e;
     return;
   }
   if (!any) return;
@@ -380,8 +384,8 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
     const float spk = lif_step(s, c, I, (sb >> k) & 1u);
     const bool fired = act && spk != 0.0f;
     nspk += (fired && t0 + k < t_stop) ? 1u : 0u;
-    emit(s, fired, fired && transmit, lane, pre, tb, wave, nblk, rrow, slot,
-         k, evs, nev);
+    emit(s, act, fired, fired && transmit, lane, pre, tb, wave, nblk, rrow,
+         slot, k, evs, nev);
     rrow = (rrow + 1 == s.raster_len) ? 0 : rrow + 1;
     slot = (slot + 1 == FIS_RING_LEN) ? 0 : slot + 1;
   }

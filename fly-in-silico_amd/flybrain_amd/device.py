@@ -309,14 +309,16 @@ class DeviceSim:
                 self._resend(old)
 
     def _resend(self, old):
-        """The last step's spikes were scattered with the old transmit flag;
-        the reference masks them with the flag current at the next step
-        (sim.py:910-918).  Correct the ring slot they went to."""
+        """The last step's spikes were sent with the old transmit flag; the
+        reference masks them with the flag current at the next step
+        (sim.py:910-918).  Correct where they went: the ring slot (push
+        road, or the pull road right after a restore, when they were seeded
+        into the ring) or the last step's spike record (pull road)."""
         changed = np.flatnonzero(((old ^ self.fl_host) & L.OUT_OFF).reshape(-1))
         if changed.size == 0:
             return
         idx = torch.from_numpy(changed.astype(np.int32)).to(self.device)
-        if self.road == "pull":
+        if self.road == "pull" and self.t >= self.ring_until:
             L.check(self.lib.fis_pull_resend(self._sp, _ptr(idx),
                                              int(idx.numel()), _stream()),
                     "fis_pull_resend")

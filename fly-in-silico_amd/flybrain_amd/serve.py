@@ -132,7 +132,6 @@ class Server:
     def __init__(self, net: DeviceNet, slots=256, window=20, road="auto",
                  hbm_fraction=0.5, max_drive=None, graph_windows=0):
         self.net = net
-        n = net.n
         cap = self.capacity(net, road, hbm_fraction)
         self.slots = int(min(slots, cap))
         if self.slots < 1:
@@ -148,19 +147,25 @@ class Server:
         # are skipped by the pull road's windows
         self.sim.track_idle_words()
         self.sim.set_trial_end(np.arange(self.slots), np.zeros(self.slots))
-        self.max_drive = int(max_drive or n)
-        # drive table of fixed size (one rate per slot), so a captured
-        # graph stays valid when requests come and go
-        self._cap = self.slots * self.max_drive if max_drive else 0
-        self._table = None
-        self._rates = np.zeros((1, self.slots), np.float32)
+        # drive table of fixed layout: slot s owns entries
+        # [s * max_drive, (s + 1) * max_drive) and rate group s; group
+        # `slots` has rate 0 and pads the unused entries.  Admission writes
+        # one slot's range, retirement zeroes its rate, and the table's
+        # pointers and size never change (a captured graph stays valid).
+        self.max_drive = int(max_drive or 256)
+        md = self.max_drive
+        ent = np.zeros((self.slots * md, 2), np.uint32)
+        ent[:, 0] = np.arange(self.slots * md) // md  # neuron 0, own trial
+        ent[:, 1] = self.slots
+        self._ent_host = ent
+        self._rate_host = np.zeros(self.slots + 1, np.float32)
+        dev = self.sim.device
+        self._entries = torch.from_numpy(ent.view(np.int32).reshape(-1)).to(dev)
+        self._rate = torch.zeros(self.slots + 1, dtype=torch.float32,
+                                 device=dev)
         self.sched = Scheduler(self.slots)
         self._graph = None
         self.graph_windows = int(graph_windows)
-        if self.graph_windows and not max_drive:
-            raise ValueError("graph replay needs a fixed-size drive table "
-                             "(max_drive)")
-        self._drive = {}  # slot -> neuron indices
 
     @staticmethod
     def capacity(net, road="auto", hbm_fraction=0.5):
@@ -169,47 +174,27 @@ class Server:
         return int(free * hbm_fraction // per)
 
     # ------------------------------------------------------------ table
-    def _build_table(self):
-        ent = []
-        for slot, ne in self._drive.items():
-            idx = ne.astype(np.int64) * self.sim.bstride + slot
-            e = np.empty((idx.size, 2), np.uint32)
-            e[:, 0] = idx
-            e[:, 1] = slot
-            ent.append(e)
-        ent = np.concatenate(ent) if ent else np.zeros((0, 2), np.uint32)
-        dev = self.sim.device
-        if self._cap:
-            if ent.shape[0] > self._cap:
-                raise ValueError("more driven neuron-trials than max_drive")
-            pad = np.zeros((self._cap - ent.shape[0], 2), np.uint32)
-            pad[:, 1] = self.slots  # a zero-rate group
-            ent = np.concatenate([ent, pad])
-            rates = np.zeros((1, self.slots + 1), np.float32)
-            rates[0, :self.slots] = self._rates[0]
-        else:
-            rates = self._rates
-        if self._table is None or self._table["n"] != ent.shape[0] or \
-                self._table["g"] != rates.shape[1]:
-            self._table = {
-                "entries": torch.zeros(max(1, ent.size), dtype=torch.int32,
-                                       device=dev),
-                "rate": torch.zeros(rates.size, dtype=torch.float32,
-                                    device=dev),
-                "n": ent.shape[0], "g": rates.shape[1]}
-            self._graph = None
-        if ent.size:
-            self._table["entries"][:ent.size].copy_(
-                torch.from_numpy(ent.view(np.int32).reshape(-1)))
-        self._table["rate"].copy_(torch.from_numpy(rates.reshape(-1)))
+    def _set_slot_drive(self, slot, drive, rate_hz):
+        md = self.max_drive
+        r0 = slot * md
+        rows = self._ent_host[r0:r0 + md]
+        rows[:, 0] = slot
+        rows[:, 1] = self.slots
+        rows[:drive.size, 0] = drive.astype(np.int64) * self.sim.bstride + slot
+        rows[:drive.size, 1] = slot
+        self._entries[2 * r0:2 * (r0 + md)].copy_(
+            torch.from_numpy(rows.view(np.int32).reshape(-1)))
+        self._rate_host[slot] = np.float32(rate_hz)
+
+    def _push_rates(self):
+        self._rate.copy_(torch.from_numpy(self._rate_host))
 
     def _window(self, W):
-        s, tb = self.sim, self._table
-        if tb is not None and tb["n"]:
-            L.check(s.lib.fis_drive_table(
-                s._sp, ctypes_ptr(tb["entries"]), tb["n"],
-                ctypes_ptr(tb["rate"]), tb["g"], 1, 1 << 30, W,
-                ctypes_stream()), "fis_drive_table")
+        s = self.sim
+        L.check(s.lib.fis_drive_table(
+            s._sp, ctypes_ptr(self._entries), self.slots * self.max_drive,
+            ctypes_ptr(self._rate), self.slots + 1, 1, 1 << 30, W,
+            ctypes_stream()), "fis_drive_table")
         s.step_launch(W)
 
     def _capture(self):
@@ -259,19 +244,17 @@ class Server:
             out_tr.append(np.full(sil.size, slot))
             free_ne.append(drv)
             free_tr.append(np.full(drv.size, slot))
-            self._drive[slot] = drv
-            self._rates[0, slot] = np.float32(req.rate_hz)
+            self._set_slot_drive(slot, drv, req.rate_hz)
         s.restart_trials(slots, [int(r.seed) for _, r in admits],
                          [s.t + int(r.steps) for _, r in admits],
                          out_neurons=np.concatenate(out_ne),
                          out_trials=np.concatenate(out_tr),
                          free_neurons=np.concatenate(free_ne),
                          free_trials=np.concatenate(free_tr))
-        self._build_table()
+        self._push_rates()
 
     def _retire(self, slot):
-        self._drive.pop(slot, None)
-        self._rates[0, slot] = 0.0
+        self._rate_host[slot] = 0.0
         self.sim.set_trial_end([slot], [0])
 
     # ------------------------------------------------------------ loop
@@ -311,7 +294,7 @@ class Server:
                 self._retire(slot)
                 if on_result is not None:
                     on_result(res)
-            self._build_table()
+            self._push_rates()
         sync()
         wall = time.perf_counter() - start
         return results, summarize(results, wall, self.slots, busy_windows,

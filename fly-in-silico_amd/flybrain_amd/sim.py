@@ -22,6 +22,7 @@ Differences a caller can observe, all documented where they arise:
 from __future__ import annotations
 
 import json
+import os
 import time
 from dataclasses import asdict, dataclass, field
 from pathlib import Path
@@ -47,7 +48,10 @@ SPMV_CSR = "эталонный"
 SPMV_EVENT = "событийный"
 SPMV_HIP = "hip"
 SPMV_CHOICES = (SPMV_HIP, SPMV_CSR, SPMV_EVENT)
-SPMV_DEFAULT = SPMV_HIP
+# The default road comes from the environment like the reference's
+# (sim.py:235); every accepted name runs the HIP road, and an unknown name
+# is refused when an Engine is built.
+SPMV_DEFAULT = os.environ.get("FLYBRAIN_SPMV", SPMV_HIP)
 
 FAST = "быстрый"
 REFERENCE = "эталонный"
@@ -82,6 +86,9 @@ class RunConfig:
     params: dict = field(default_factory=lambda: dict(MODEL_PARAMS))
     speed: str = FAST
     spmv: str | None = None
+    # extension: the engine's road ("auto", "push" or "pull"); results are
+    # the same on either
+    road: str = "auto"
 
 
 class Result:
@@ -178,7 +185,8 @@ class Engine:
     MASK_TENSORS = ("m_fire", "m_out", "m_in", "m_hold", "v_hold")
 
     def __init__(self, weights, n, params=None, device="cuda", batch=1,
-                 seed=0, no_refractory=None, spmv=None, allow_inexact=False):
+                 seed=0, no_refractory=None, spmv=None, allow_inexact=False,
+                 road="auto"):
         p = dict(MODEL_PARAMS)
         if params:
             p.update(params)
@@ -224,9 +232,13 @@ class Engine:
                     if torch.is_tensor(no_refractory)
                     else np.asarray(no_refractory, dtype=np.int64))
         self._free = free
+        # road: "push" (event scatter into the int32 ring) or "pull"
+        # (per-window gather from spike records, 64..1024 trials); "auto"
+        # takes the pull road from 64 trials up.  Same results either way.
         self._sim = DeviceSim(self._net, self.batch, params=p,
                               seeds=[int(seed) & ((1 << 64) - 1)] * self.batch,
-                              no_refractory=free, spikes_out=True)
+                              no_refractory=free, spikes_out=True, road=road)
+        self.road = self._sim.road
         self.gen = _Gen(self, seed)
         self.refrac_steps = torch.full((self.n,), float(round(p["tRefrac"] / DT)),
                                        dtype=torch.float32, device=dev)
@@ -302,8 +314,11 @@ class Engine:
 
     # --- road report ------------------------------------------------------
     def spmv_info(self):
+        road = ("libfis event scatter, int32 ring (gfx950)"
+                if self.road == "push" else
+                "libfis windowed gather from spike records (gfx950)")
         out = {"умножение связей": self.spmv,
-               "road": "libfis event scatter, int32 ring (gfx950)",
+               "road": road,
                "requested": self._spmv_requested,
                "побитовое равенство обеспечено": bool(self._spmv_exact["ok"]),
                "условия равенства": self._spmv_exact}
@@ -364,8 +379,7 @@ class Engine:
     def _launch_step(self, rates):
         s = self._sim
         s.drive_rates(rates, 1)
-        s.update(1)
-        s.scatter()
+        s.step_launch(1)
 
     def use_graphs(self, enable=True):
         """Capture one step (drive + update + scatter) into a HIP graph.
@@ -407,6 +421,7 @@ class Engine:
         s = self._sim
         names = ["v", "g", "st", "ring", "sbits", "ev_count", "clock",
                  "spikes_out"]
+        names += [k for k in ("rec", "act") if getattr(s, k) is not None]
         return ({k: getattr(s, k).clone() for k in names}, s.t)
 
     def _device_restore(self, snap):
@@ -429,7 +444,9 @@ class Engine:
             # the kernel arguments changed since capture: capture again
             self._graph = self._graphs = None
             self.use_graphs()
-        if self._graph is not None:
+        # (on the pull road the 20 steps after a restore also read the
+        # restored ring: launched directly, the graph does not carry that)
+        if self._graph is not None and self._sim.t >= self._sim.ring_until:
             if r.data_ptr() != self._rates.data_ptr():
                 self._rates.copy_(r)
             self._graph.replay()
@@ -443,8 +460,7 @@ class Engine:
         self.calls += 1
         s = self._sim
         s.drive_tape(tape, 1)
-        s.update(1)
-        s.scatter()
+        s.step_launch(1)
         return self.spikes
 
     def advance(self, steps, drive=None):
@@ -568,11 +584,17 @@ class Engine:
             ring[u % L.RING_LEN, :, :self.batch] = I.T.astype(np.int32)
         s.ring.copy_(torch.from_numpy(ring.reshape(-1)))
         s.seed_ring(T % L.RING_LEN)
+        if s.road == "pull":
+            # everything in flight is now in the int32 ring, which the next
+            # 20 steps consume; the spike records hold nothing older
+            s.rec.zero_()
+            s.act.zero_()
+            s.ring_until = T + L.RING_LEN
 
 
 def build_engine(weights=None, neurons=None, silenced=None, params=None,
                  device=None, batch=1, seed=0, no_refractory=None,
-                 graphs=True, spmv=None, allow_inexact=False):
+                 graphs=True, spmv=None, allow_inexact=False, road="auto"):
     """Load what is needed and return a ready :class:`Engine`
     (sim.py:956-987).  ``weights``: reference-format sparse COO W[post, pre],
     a ``{indices, values, shape}`` dict, or a Connectome."""
@@ -592,7 +614,7 @@ def build_engine(weights=None, neurons=None, silenced=None, params=None,
                                      dtype=np.int64))
     eng = Engine(conn, n, params=params, device=device, batch=batch, seed=seed,
                  no_refractory=no_refractory, spmv=spmv,
-                 allow_inexact=allow_inexact)
+                 allow_inexact=allow_inexact, road=road)
     if graphs:
         eng.use_graphs()
     return eng
@@ -672,7 +694,8 @@ def run(stimuli, silenced=None, config: RunConfig | None = None, neurons=None,
     engine = build_engine(weights=weights, neurons=neurons, silenced=silenced,
                           params=p, device=device, batch=B, seed=config.seed,
                           no_refractory=stim_all, graphs=False,
-                          spmv=getattr(config, "spmv", None))
+                          spmv=getattr(config, "spmv", None),
+                          road=getattr(config, "road", "auto"))
     sim = engine._sim
     sim.pop_counts = torch.zeros(n, dtype=torch.int32, device=sim.device)
     sim._sync_struct()

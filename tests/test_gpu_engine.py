@@ -16,7 +16,8 @@ from gpu_helpers import tiny_conn
 pytestmark = pytest.mark.gpu
 
 
-def test_engine_step_matches_oracle_philox():
+@pytest.mark.parametrize("road", ["push", "pull"])
+def test_engine_step_matches_oracle_philox(road):
     """Engine.step(rates) with masks changing, B=3, against the oracle fed
     the same Philox stream; the graph road gives the same bytes."""
     conn = C.synthetic(400, None, mean_out=40, count_exp=1.2, n_groups=3,
@@ -28,7 +29,8 @@ def test_engine_step_matches_oracle_philox():
     rates[1, drv[::2]] = 2500.0
     outs = []
     for graphs in (False, True):
-        eng = S.Engine(conn, n, batch=B, seed=1234, no_refractory=drv)
+        eng = S.Engine(conn, n, batch=B, seed=1234, no_refractory=drv,
+                       road=road)
         if graphs:
             assert eng.use_graphs()
         ora = O.OracleEngine(conn, B, no_refractory=drv,
@@ -58,7 +60,8 @@ def test_engine_step_matches_oracle_philox():
     assert np.array_equal(outs[0], outs[1])
 
 
-def test_snapshot_restore_continues_bit_exact():
+@pytest.mark.parametrize("road", ["push", "pull"])
+def test_snapshot_restore_continues_bit_exact(road):
     """Resume from a snapshot into a fresh engine: the continuation is the
     same bytes (probe/resume_state.py); dropping the ring breaks it."""
     conn = C.synthetic(500, None, mean_out=40, count_exp=1.2, n_groups=3,
@@ -67,27 +70,28 @@ def test_snapshot_restore_continues_bit_exact():
     drv = np.arange(0, n, 11)
     rates = torch.zeros(B, n, device="cuda")
     rates[:, drv] = 900.0
-    a = S.Engine(conn, n, batch=B, seed=77, no_refractory=drv)
+    a = S.Engine(conn, n, batch=B, seed=77, no_refractory=drv, road=road)
     for _ in range(137):
         a.step(rates)
     snap = a.snapshot()
     ref = []
     for _ in range(120):
         ref.append(a.step(rates).cpu().numpy().copy())
-    b = S.Engine(conn, n, batch=B, seed=5, no_refractory=drv)
+    b = S.Engine(conn, n, batch=B, seed=5, no_refractory=drv, road=road)
     done = b.restore(snap)
     assert {"v", "conductance", "spikes", "refrac", "delay_buffer", "ptr",
             "gen"} <= set(done)
     for k in range(120):
         assert np.array_equal(b.step(rates).cpu().numpy(), ref[k]), k
-    c = S.Engine(conn, n, batch=B, seed=5, no_refractory=drv)
+    c = S.Engine(conn, n, batch=B, seed=5, no_refractory=drv, road=road)
     c.restore(snap, drop=("delay_buffer",))
     differs = any(not np.array_equal(c.step(rates).cpu().numpy(), ref[k])
                   for k in range(120))
     assert differs, "dropping the ring must change the continuation"
 
 
-def test_run_matches_reference_golden(golden):
+@pytest.mark.parametrize("road", ["push", "pull"])
+def test_run_matches_reference_golden(golden, road):
     """run() fed the reference's recorded Bernoulli tape: per-bin int16
     raster and per-neuron counts equal the reference's run()."""
     z = golden("run_sched.npz")
@@ -102,7 +106,7 @@ def test_run_matches_reference_golden(golden):
     bits = np.unpackbits(tape, axis=-1)[..., :n]
     t, b, i = np.nonzero(bits)
     cfg = S.RunConfig(t_run_s=float(z["t_run_s"]), bin_ms=float(z["bin_ms"]),
-                      n_trials=B, seed=0, device="cuda")
+                      n_trials=B, seed=0, device="cuda", road=road)
     got_bins = []
     res = S.run(st, silenced=z["silenced"], config=cfg, weights=conn,
                 tape=(t, i, b), on_bin=lambda k, ms, c: got_bins.append(k))
@@ -119,7 +123,8 @@ def test_engine_refuses_cpu_and_bad_roads():
         S.Engine(conn, 50, spmv="nope")
 
 
-def test_tiny_masks_through_engine(golden):
+@pytest.mark.parametrize("road", ["push", "pull"])
+def test_tiny_masks_through_engine(golden, road):
     """The Engine surface on the tiny fixture (step_tape, set_masks): v,
     g and spikes fingerprints equal the reference's at every step."""
     import hashlib
@@ -131,7 +136,8 @@ def test_tiny_masks_through_engine(golden):
     conn = tiny_conn(z)
     n, B = conn.n, int(z["batch"])
     bits = np.unpackbits(z["tape_bits"], axis=-1)[..., :n]
-    eng = S.Engine(conn, n, batch=B, no_refractory=z["no_refractory"])
+    eng = S.Engine(conn, n, batch=B, no_refractory=z["no_refractory"],
+                   road=road)
     t_, b_, i_ = np.nonzero(bits)
     tape = Tape(eng._sim, t_, i_, b_)
     sched = mask_schedule(z)
@@ -149,7 +155,8 @@ def test_tiny_masks_through_engine(golden):
         assert fp(db) == z["fps"][t][4], t
 
 
-def test_out_mask_masks_the_previous_steps_spikes():
+@pytest.mark.parametrize("road", ["push", "pull"])
+def test_out_mask_masks_the_previous_steps_spikes(road):
     """m_out set (or lifted) between steps applies to the spikes of the step
     just taken, as rec = W @ (spikes * m_out) in the reference's next step
     (sim.py:910-918): a spike at step T-1 followed by set_masks(out=...)
@@ -168,7 +175,7 @@ def test_out_mask_masks_the_previous_steps_spikes():
         for i, b in pairs:
             bits[t, b, i] = 1
     t_, b_, i_ = np.nonzero(bits)
-    eng = S.Engine(conn, n, batch=B)
+    eng = S.Engine(conn, n, batch=B, road=road)
     tape = Tape(eng._sim, t_, i_, b_)
     ora = O.OracleEngine(conn, B)
     masks = {10: dict(out=[hub]), 40: dict(out=[]), 70: dict(out=[hub])}
@@ -187,3 +194,34 @@ def test_out_mask_masks_the_previous_steps_spikes():
                                       err_msg="t=%d" % t)
     # the hub did spike at each kicked step (the test exercised the fix)
     assert ora.conductance.any()
+
+
+def test_roads_agree_after_restore_and_out_mask():
+    """Restore a snapshot into a push-road and a pull-road engine, switch
+    m_out on spiking hubs at once (their last spikes sit in the restored
+    ring), then step: both roads give the same spikes and state."""
+    conn = C.synthetic(500, None, mean_out=50, count_exp=1.15, n_groups=3,
+                       seed=41)
+    n, B = conn.n, 3
+    drv = np.arange(0, n, 7)
+    rates = torch.zeros(B, n, device="cuda")
+    rates[:, drv] = 1500.0
+    src = S.Engine(conn, n, batch=B, seed=9, no_refractory=drv, road="push")
+    for _ in range(83):
+        src.step(rates)
+    snap = src.snapshot()
+    hubs = list(np.argsort(-conn.degree())[:30])
+    engines = [S.Engine(conn, n, batch=B, seed=1, no_refractory=drv,
+                        road=road) for road in ("push", "pull")]
+    for eng in engines:
+        eng.restore(snap)
+        eng.set_masks(out=hubs)
+        eng.use_graphs()
+    assert engines[1].road == "pull"
+    for t in range(60):
+        a, b = (e.step(rates).cpu().numpy().copy() for e in engines)
+        assert np.array_equal(a, b), t
+    np.testing.assert_array_equal(engines[0].v.cpu().numpy().view(np.uint32),
+                                  engines[1].v.cpu().numpy().view(np.uint32))
+    np.testing.assert_array_equal(
+        engines[0].delay_buffer.numpy(), engines[1].delay_buffer.numpy())

@@ -209,8 +209,17 @@ def test_pull_road_uneven_windows_and_out_masks(batch):
         for j in range(W):
             ora.step_rates(rates)
             for r in sims:
-                assert np.array_equal(got[r][j], ora.spikes.astype(bool)), \
-                    "%s road: raster differs at step %d" % (r, t + j)
+                want = ora.spikes.astype(bool)
+                if not np.array_equal(got[r][j], want):
+                    bt, nn = np.nonzero(got[r][j] != want)
+                    driven = [bool(rates[b, i] > 0) for b, i in zip(bt, nn)]
+                    pytest.fail(
+                        "%s road: raster differs at step %d: %d neuron-trials"
+                        ", first (trial, neuron, got, driven) %s"
+                        % (r, t + j, bt.size,
+                           list(zip(bt[:8].tolist(), nn[:8].tolist(),
+                                    got[r][j][bt[:8], nn[:8]].tolist(),
+                                    driven[:8]))))
         t += W
         for r, (s, _) in sims.items():
             compare_state(s, ora, "%s road, step %d" % (r, t))
@@ -273,3 +282,49 @@ def test_ev_spmv_twin_matches_dense():
         assert rc == 0
         np.testing.assert_array_equal(out.cpu().numpy(), want.astype(np.float32))
     assert lib.fis_ev_spmv_abi() == 3
+
+
+@pytest.mark.parametrize("road,batch", [("push", 64), ("push", 3),
+                                        ("push", 130), ("pull", 64),
+                                        ("pull", 200)])
+def test_outputs_stay_inside_their_buffers(road, batch):
+    """509 neurons: the last workgroup has waves past the last neuron.
+    Guard words placed right after every buffer a launch writes per wave or
+    per workgroup (events, event counts, raster, spike records, state) are
+    still intact after windows on either road."""
+    conn = C.synthetic(509, None, mean_out=45, count_exp=1.15, n_groups=4,
+                       seed=23)
+    n = conn.n
+    rng = np.random.default_rng(batch)
+    drv = rng.choice(n, 120, replace=False)
+    net = DeviceNet(conn)
+    sim = DeviceSim(net, batch, seeds=rng.integers(0, 2**63, size=batch,
+                                                   dtype=np.uint64),
+                    raster_len=20, road=road, counts=True)
+    sentinel = -0x5A5A5A5A5A5A5A5B
+    guards = {}
+    names = ["events", "ev_count", "raster", "v", "g", "sbits", "counts",
+             "ring"]
+    if road == "pull":
+        names += ["rec", "act"]
+    for name in names:
+        t = getattr(sim, name)
+        nbytes = t.numel() * t.element_size()
+        words = (nbytes + 7) // 8 + 512  # 4 KB of guard after the data
+        big = torch.full((words,), sentinel, dtype=torch.int64,
+                         device=t.device)
+        body = big.view(torch.uint8)[:nbytes].view(t.dtype)
+        body.copy_(t)
+        setattr(sim, name, body)
+        guards[name] = (big, (nbytes + 7) // 8)
+    sim._sync_struct()
+    ne = np.repeat(drv, batch)
+    tr = np.tile(np.arange(batch), drv.size)
+    table = DriveTable(sim, ne, tr, tr, np.full(batch, 900.0, np.float32))
+    for W in (20, 20, 7, 13, 20):
+        sim.window(W, table)
+    torch.cuda.synchronize()
+    assert sim.read_raster(sim.t - 20, sim.t).any()
+    for name, (big, start) in guards.items():
+        tail = big[start:].cpu().numpy()  # words after the last data byte
+        assert (tail == sentinel).all(), name
