@@ -57,8 +57,9 @@ def kernel_times(runner, windows):
 
     s = runner.sim
     W = runner.window
-    stages = [("drive", lambda: runner.table is not None
-               and s.drive_table(runner.table, W))]
+    stages = [] if runner.fused else [
+        ("drive", lambda: runner.table is not None
+         and s.drive_table(runner.table, W))]
     if s.road == "pull":
         stages.append(("pull", lambda: s.pull(W)))
     else:
@@ -117,7 +118,8 @@ def serve_bench(net, conn, args):
            "requests": len(reqs),
            "mean_steps": float(np.mean([r.steps for r in reqs]))}
     runs = {}
-    for name, kw in (("batched", dict(slots=args.serve_slots)),
+    for name, kw in (("batched", dict(slots=args.serve_slots,
+                                      shard=args.serve_shard)),
                      ("b1_back_to_back", dict(slots=1, road="push"))):
         srv = Server(net, max_drive=200, graph_windows=args.graph_windows,
                      **kw)
@@ -125,6 +127,7 @@ def serve_bench(net, conn, args):
         res, st = srv.serve(copies(), realtime=True)
         runs[name] = {r.rid: r.counts for r in res}
         st["road"] = srv.road
+        st["shard"] = srv.shard
         out[name] = st
         log("serve %s: %s" % (name, st))
         del srv
@@ -252,6 +255,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0,
                     help="Poisson arrival rate of requests per second")
     ap.add_argument("--serve-slots", type=int, default=256)
+    ap.add_argument("--serve-shard", type=int, default=32,
+                    help="slots per shard (one DeviceSim each)")
     ap.add_argument("--road", choices=("auto", "push", "pull"), default="auto",
                     help="auto: pull road for 64..1024 trials, push below")
     ap.add_argument("--no-graph", action="store_true",

@@ -261,6 +261,25 @@ __device__ __forceinline__ void load_cell(const fis_sim& s, int idx, float& v,
   if (sb) s.sbits[idx] = 0u;  // the window's kicks are consumed
 }
 
+// The window's kicks of a table-driven neuron-trial (fused drive): the
+// bits k_drive_table would set for it.
+__device__ __forceinline__ uint32_t fused_kicks(const fis_sim& s, int idx,
+                                                int neuron, int trial,
+                                                int64_t t0, int W) {
+  const int gr = s.drive_group[idx];
+  if (gr < 0) return 0u;
+  const float ps = s.prob_scale;
+  const int seg_steps = s.drive_seg_steps, n_seg = s.drive_n_seg;
+  const int n_groups = s.drive_n_groups;
+  const float* rate = s.drive_rate;
+  return window_bits(t0 - origin_of(s, trial), W, (uint32_t)neuron,
+                     s.trial_ids[trial], s.seeds[trial], [&](int64_t t) {
+                       int64_t seg = t / seg_steps;
+                       if (seg >= n_seg) seg = n_seg - 1;
+                       return rate[seg * n_groups + gr] * ps;
+                     });
+}
+
 __device__ __forceinline__ Cell make_cell(const fis_sim& s, uint32_t fl,
                                           float vh, uint32_t st, float v,
                                           float g) {
@@ -346,6 +365,7 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
   float v = 0.0f, g = 0.0f, vh = 0.0f;
   uint32_t st = 0, fl = FIS_FIRE_OFF, sb = 0;
   if (act) load_cell(s, idx, v, g, st, fl, sb, vh);
+  if (act && s.drive_group) sb |= fused_kicks(s, idx, pre, tb, t0, W);
 
   Cell c = make_cell(s, fl, vh, st, v, g);
   const bool transmit = act && !(fl & FIS_OUT_OFF);
@@ -913,6 +933,9 @@ int check_sim(const fis_sim* s) {
     return (int)hipErrorInvalidValue;
   if (s->raster && s->raster_len < FIS_MAX_WINDOW)
     return (int)hipErrorInvalidValue;
+  if (s->drive_group && (!s->drive_rate || s->drive_n_groups <= 0 ||
+                         s->drive_n_seg <= 0 || s->drive_seg_steps <= 0))
+    return (int)hipErrorInvalidValue;
   if ((int64_t)s->n * s->bstride >= ((int64_t)1 << 31) - kBlock)
     return (int)hipErrorInvalidValue;
   return 0;
@@ -926,6 +949,11 @@ int check_pull(const fis_sim* s) {
   if (!s->in_offsets || !s->in_cols || !s->in_vals || !s->rec || !s->act ||
       !s->order)
     return (int)hipErrorInvalidValue;
+  // the pull road draws a drive table in its own launch: the window kernel
+  // is bound by vector-instruction issue, and Philox on the few driven
+  // lanes of a wave would cost it more than a separate, fully occupied
+  // drive launch
+  if (s->drive_group) return (int)hipErrorInvalidValue;
   return 0;
 }
 

@@ -20,7 +20,7 @@ CSRC = PKG.parent / "csrc"
 SOURCES = [CSRC / "lif.hip"]
 HEADER = ROOT / "include" / "fis.h"
 LIB_PATH = PKG / "libfis.so"
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 MAX_WINDOW = 20
 RING_LEN = 20
@@ -85,6 +85,9 @@ class FisSim(ctypes.Structure):
         ("in_offsets", P), ("in_cols", P), ("in_vals", P),
         ("rec", P), ("act", P), ("order", P),
         ("t_origin", P), ("t_end", P), ("word_end", P),
+        ("drive_group", P), ("drive_rate", P),
+        ("drive_n_groups", ctypes.c_int32), ("drive_n_seg", ctypes.c_int32),
+        ("drive_seg_steps", ctypes.c_int32),
     ]
 
 

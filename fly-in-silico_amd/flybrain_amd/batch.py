@@ -7,9 +7,10 @@ own Philox seed, drive table entries and m_out masks; trials never
 interact, so trial ``b`` of a batch produces the same raster as the same
 trial run alone (tests check this bit for bit).
 
-One window is the drive-table launch plus the road's launches: on the
-push road update (W steps) and scatter, on the pull road (64..1,024
-trials, the default there) one fused gather+update launch.
+One window is the road's launches: on the push road update (W steps,
+drawing the drive table's Philox kicks itself) and scatter; on the pull
+road (64..1,024 trials, the default there) a drive-table launch and one
+fused gather+update launch.
 ``graph_windows`` windows are captured into one graph.  The step
 counter lives in device memory, so the same graph serves every replay.
 """
@@ -27,7 +28,7 @@ from .workloads import Workload
 class BatchRunner:
     def __init__(self, net: DeviceNet, wl: Workload, params=None,
                  raster_len=0, counts=True, pop_counts=False, window=20,
-                 road="auto"):
+                 road="auto", fused_drive=True):
         if not 1 <= window <= L.MAX_WINDOW:
             raise ValueError("window must be 1..%d" % L.MAX_WINDOW)
         self.wl = wl
@@ -46,6 +47,13 @@ class BatchRunner:
         self.table = DriveTable(sim, wl.drive_neuron, wl.drive_trial,
                                 wl.drive_group, wl.rate_hz, wl.seg_steps) \
             if wl.n_drive else None
+        # the update launches draw the table's kicks themselves: one launch
+        # less per window (the same bits as the drive-table launch)
+        self.fused = bool(fused_drive and self.table is not None
+                          and sim.road == "push"
+                          and self.table.n_groups <= 32767)
+        if self.fused:
+            sim.fuse_drive(self.table)
         self._graph = None
         self._graph_windows = 0
 
@@ -55,7 +63,7 @@ class BatchRunner:
 
     def _window(self, W):
         s = self.sim
-        if self.table is not None:
+        if self.table is not None and not self.fused:
             s.drive_table(self.table, W)
         s.step_launch(W)
 

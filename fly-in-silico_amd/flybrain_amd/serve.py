@@ -15,7 +15,10 @@ of trial slots:
     are counted until its last step (``t_end``);
   * all admitted requests advance together, one window of 20 steps per
     launch, and a request whose last step has run is read out (its column
-    of spike counts) and its slot freed for the next arrival.
+    of spike counts) and its slot freed for the next arrival;
+  * slots are held in shards (64 trials each by default, one DeviceSim per
+    shard) and a window launches only the shards holding a live request,
+    so a partly filled server costs what its live shards cost.
 
 Trials never interact, so a request's result is bit-for-bit the result of
 the same request run alone; ``tests/test_gpu_serve.py`` checks this.
@@ -126,44 +129,93 @@ def bytes_per_trial(n, road):
     return per * int(n)
 
 
+class _Shard:
+    """One DeviceSim of ``size`` slots and its drive table.
+
+    The drive table has a fixed layout: slot s owns entries
+    [s * max_drive, (s + 1) * max_drive) and rate group s; group ``size``
+    has rate 0 and pads the unused entries.  Admission writes one slot's
+    range, retirement zeroes its rate, and the table's pointers and size
+    never change (a captured graph stays valid).
+    """
+
+    def __init__(self, net, size, road, max_drive):
+        self.size = int(size)
+        self.sim = DeviceSim(net, self.size, counts=True, road=road,
+                             seeds=np.zeros(self.size, np.uint64),
+                             trial_ids=np.zeros(self.size, np.uint32))
+        # idle slots count nothing; on the pull road, 64-slot words with no
+        # live request are skipped
+        self.sim.track_idle_words()
+        self.sim.set_trial_end(np.arange(self.size), np.zeros(self.size))
+        self.max_drive = md = int(max_drive)
+        ent = np.zeros((self.size * md, 2), np.uint32)
+        ent[:, 0] = np.arange(self.size * md) // md  # neuron 0, own trial
+        ent[:, 1] = self.size
+        self._ent_host = ent
+        self._rate_host = np.zeros(self.size + 1, np.float32)
+        dev = self.sim.device
+        self._entries = torch.from_numpy(ent.view(np.int32).reshape(-1)).to(dev)
+        self._rate = torch.zeros(self.size + 1, dtype=torch.float32,
+                                 device=dev)
+        self.live = 0
+
+    def set_slot_drive(self, slot, drive, rate_hz):
+        md = self.max_drive
+        r0 = slot * md
+        rows = self._ent_host[r0:r0 + md]
+        rows[:, 0] = slot
+        rows[:, 1] = self.size
+        rows[:drive.size, 0] = drive.astype(np.int64) * self.sim.bstride + slot
+        rows[:drive.size, 1] = slot
+        self._entries[2 * r0:2 * (r0 + md)].copy_(
+            torch.from_numpy(rows.view(np.int32).reshape(-1)))
+        self._rate_host[slot] = np.float32(rate_hz)
+
+    def push_rates(self):
+        self._rate.copy_(torch.from_numpy(self._rate_host))
+
+    def window(self, W):
+        s = self.sim
+        L.check(s.lib.fis_drive_table(
+            s._sp, ctypes_ptr(self._entries), self.size * self.max_drive,
+            ctypes_ptr(self._rate), self.size + 1, 1, 1 << 30, W,
+            ctypes_stream()), "fis_drive_table")
+        s.step_launch(W)
+
+
 class Server:
-    """A resident batch of ``slots`` trials serving SimRequests."""
+    """Slots for SimRequests, held in shards of ``shard`` trials.
+
+    Each shard is its own DeviceSim; a window launches only the shards
+    that hold a live request, so the GPU time of a window follows the live
+    load rather than the capacity.  The default shard of 64 trials is the
+    pull road's smallest stride (one 64-bit trial word).
+    """
 
     def __init__(self, net: DeviceNet, slots=256, window=20, road="auto",
-                 hbm_fraction=0.5, max_drive=None, graph_windows=0):
+                 hbm_fraction=0.5, max_drive=None, graph_windows=0,
+                 shard=64):
         self.net = net
         cap = self.capacity(net, road, hbm_fraction)
         self.slots = int(min(slots, cap))
         if self.slots < 1:
             raise MemoryError("no HBM for a single trial slot")
-        self.road = pick_road(self.slots, road)
         self.window = int(window)
         if not 1 <= self.window <= L.MAX_WINDOW:
             raise ValueError("window must be 1..%d" % L.MAX_WINDOW)
-        self.sim = DeviceSim(net, self.slots, counts=True, road=self.road,
-                             seeds=np.zeros(self.slots, np.uint64),
-                             trial_ids=np.zeros(self.slots, np.uint32))
-        # idle slots count nothing, and 64-slot words with no live request
-        # are skipped by the pull road's windows
-        self.sim.track_idle_words()
-        self.sim.set_trial_end(np.arange(self.slots), np.zeros(self.slots))
-        # drive table of fixed layout: slot s owns entries
-        # [s * max_drive, (s + 1) * max_drive) and rate group s; group
-        # `slots` has rate 0 and pads the unused entries.  Admission writes
-        # one slot's range, retirement zeroes its rate, and the table's
-        # pointers and size never change (a captured graph stays valid).
+        self.shard = int(min(shard, self.slots))
+        if self.shard < 1:
+            raise ValueError("shard must hold at least one slot")
+        self.road = pick_road(self.shard, road)
         self.max_drive = int(max_drive or 256)
-        md = self.max_drive
-        ent = np.zeros((self.slots * md, 2), np.uint32)
-        ent[:, 0] = np.arange(self.slots * md) // md  # neuron 0, own trial
-        ent[:, 1] = self.slots
-        self._ent_host = ent
-        self._rate_host = np.zeros(self.slots + 1, np.float32)
-        dev = self.sim.device
-        self._entries = torch.from_numpy(ent.view(np.int32).reshape(-1)).to(dev)
-        self._rate = torch.zeros(self.slots + 1, dtype=torch.float32,
-                                 device=dev)
+        sizes = [self.shard] * (self.slots // self.shard)
+        if self.slots % self.shard:
+            sizes.append(self.slots % self.shard)
+        self.shards = [_Shard(net, k, pick_road(k, road), self.max_drive)
+                       for k in sizes]
         self.sched = Scheduler(self.slots)
+        self.t = 0  # the server's clock: steps run since it started
         self._graph = None
         self.graph_windows = int(graph_windows)
 
@@ -173,89 +225,85 @@ class Server:
         per = bytes_per_trial(net.n, "pull" if road != "push" else "push")
         return int(free * hbm_fraction // per)
 
-    # ------------------------------------------------------------ table
-    def _set_slot_drive(self, slot, drive, rate_hz):
-        md = self.max_drive
-        r0 = slot * md
-        rows = self._ent_host[r0:r0 + md]
-        rows[:, 0] = slot
-        rows[:, 1] = self.slots
-        rows[:drive.size, 0] = drive.astype(np.int64) * self.sim.bstride + slot
-        rows[:drive.size, 1] = slot
-        self._entries[2 * r0:2 * (r0 + md)].copy_(
-            torch.from_numpy(rows.view(np.int32).reshape(-1)))
-        self._rate_host[slot] = np.float32(rate_hz)
-
-    def _push_rates(self):
-        self._rate.copy_(torch.from_numpy(self._rate_host))
-
-    def _window(self, W):
-        s = self.sim
-        L.check(s.lib.fis_drive_table(
-            s._sp, ctypes_ptr(self._entries), self.slots * self.max_drive,
-            ctypes_ptr(self._rate), self.slots + 1, 1, 1 << 30, W,
-            ctypes_stream()), "fis_drive_table")
-        s.step_launch(W)
+    def _where(self, slot):
+        return self.shards[slot // self.shard], slot % self.shard
 
     def _capture(self):
-        """One HIP graph of ``graph_windows`` windows.  The drive table,
-        the per-trial clocks and the step counter are read from device
-        memory, so the graph stays valid while requests come and go."""
+        """One HIP graph of ``graph_windows`` windows of every shard.  The
+        drive tables, the per-trial clocks and the step counters are read
+        from device memory, so the graph stays valid while requests come
+        and go."""
         g = torch.cuda.CUDAGraph()
-        t0 = self.sim.t  # capture records launches; the host clock stays
+        t0 = [sh.sim.t for sh in self.shards]  # capture only records
         try:
             with torch.cuda.graph(g):
                 for _ in range(self.graph_windows):
-                    self._window(self.window)
+                    for sh in self.shards:
+                        sh.window(self.window)
         finally:
-            self.sim.t = t0
+            for sh, t in zip(self.shards, t0):
+                sh.sim.t = t
         self._graph = g
 
     def _advance(self):
-        """Run windows up to the next completion: a whole graph when no
-        slot is free (nothing could be admitted meanwhile) and no request
-        ends inside it, else one window."""
+        """Run one window of every shard holding a live request -- or, when
+        no slot is free (nothing could be admitted meanwhile) and no request
+        ends inside it, one whole captured graph of windows."""
         per = self.graph_windows * self.window
         if per and not self.sched.free:
             left = min(t0 + req.steps for req, t0, _ in
-                       self.sched.active.values()) - self.sim.t
+                       self.sched.active.values()) - self.t
             if left >= per:
                 if self._graph is None:
                     self._capture()
                 self._graph.replay()
-                self.sim.t += per
+                for sh in self.shards:
+                    sh.sim.t += per
+                self.t += per
                 return self.graph_windows
-        self._window(self.window)
+        for sh in self.shards:
+            if sh.live:
+                sh.window(self.window)
+        self.t += self.window
         return 1
 
     # ------------------------------------------------------------ slots
     def _admit(self, admits):
-        s = self.sim
-        slots = [slot for slot, _ in admits]
-        out_ne, out_tr, free_ne, free_tr = [], [], [], []
+        by_shard = {}
         for slot, req in admits:
             drv = np.unique(np.asarray(req.drive, np.int64))
             check_rates_hz(np.array([req.rate_hz]), "request %r" % (req.rid,))
             if drv.size > self.max_drive:
                 raise ValueError("request %r drives %d neurons > max_drive"
                                  % (req.rid, drv.size))
-            sil = np.unique(np.asarray(req.silenced, np.int64))
-            out_ne.append(sil)
-            out_tr.append(np.full(sil.size, slot))
-            free_ne.append(drv)
-            free_tr.append(np.full(drv.size, slot))
-            self._set_slot_drive(slot, drv, req.rate_hz)
-        s.restart_trials(slots, [int(r.seed) for _, r in admits],
-                         [s.t + int(r.steps) for _, r in admits],
-                         out_neurons=np.concatenate(out_ne),
-                         out_trials=np.concatenate(out_tr),
-                         free_neurons=np.concatenate(free_ne),
-                         free_trials=np.concatenate(free_tr))
-        self._push_rates()
+            sh, local = self._where(slot)
+            by_shard.setdefault(id(sh), (sh, []))[1].append((local, req, drv))
+        for sh, items in by_shard.values():
+            s = sh.sim
+            out_ne, out_tr, free_ne, free_tr = [], [], [], []
+            for local, req, drv in items:
+                sil = np.unique(np.asarray(req.silenced, np.int64))
+                out_ne.append(sil)
+                out_tr.append(np.full(sil.size, local))
+                free_ne.append(drv)
+                free_tr.append(np.full(drv.size, local))
+                sh.set_slot_drive(local, drv, req.rate_hz)
+            s.restart_trials([lo for lo, _, _ in items],
+                             [int(r.seed) for _, r, _ in items],
+                             [s.t + int(r.steps) for _, r, _ in items],
+                             out_neurons=np.concatenate(out_ne),
+                             out_trials=np.concatenate(out_tr),
+                             free_neurons=np.concatenate(free_ne),
+                             free_trials=np.concatenate(free_tr))
+            sh.push_rates()
+            sh.live += len(items)
 
     def _retire(self, slot):
-        self._rate_host[slot] = 0.0
-        self.sim.set_trial_end([slot], [0])
+        sh, local = self._where(slot)
+        sh._rate_host[local] = 0.0
+        sh.sim.set_trial_end([local], [0])
+        sh.live -= 1
+        sh.push_rates()
 
     # ------------------------------------------------------------ loop
     def serve(self, requests, realtime=True, on_result=None):
@@ -271,10 +319,9 @@ class Server:
         results = []
         start = time.perf_counter()
         busy_windows = 0
-        sync = torch.cuda.synchronize
         while self.sched.queue or self.sched.busy:
             now = time.perf_counter() - start
-            admits = self.sched.admit(now, self.sim.t)
+            admits = self.sched.admit(now, self.t)
             if admits:
                 self._admit(admits)
             if not self.sched.busy:
@@ -283,19 +330,16 @@ class Server:
                     time.sleep(max(0.0, min(nxt - now, 0.001)))
                 continue
             busy_windows += self._advance()
-            done = self.sched.finished(self.sim.t)
-            if not done:
-                continue
-            for slot, req, t0, adm in done:
-                counts = self.sim.trial_counts(slot)  # synchronises
+            for slot, req, t0, adm in self.sched.finished(self.t):
+                sh, local = self._where(slot)
+                counts = sh.sim.trial_counts(local)  # synchronises
                 res = SimResult(req.rid, counts, req.steps, req.arrival_s,
                                 adm, time.perf_counter() - start, slot)
                 results.append(res)
                 self._retire(slot)
                 if on_result is not None:
                     on_result(res)
-            self._push_rates()
-        sync()
+        torch.cuda.synchronize()
         wall = time.perf_counter() - start
         return results, summarize(results, wall, self.slots, busy_windows,
                                   self.window)

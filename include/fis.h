@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define FIS_ABI_VERSION 4
+#define FIS_ABI_VERSION 5
 #define FIS_MAX_WINDOW 20   /* steps per window: delay horizon 19 + 1     */
 #define FIS_RING_LEN 20     /* int32 ring slots: rec(t-19) .. rec(t)       */
 
@@ -155,6 +155,17 @@ typedef struct {
    * records written.  Its stale records can only reach trials of the same
    * word, which fis_reset_trials clears when a slot of it restarts. */
   const int64_t* word_end;
+  /* fused drive table (nullable; push road only -- fis_pull_window refuses
+   * it): drive_group[idx] >= 0 makes fis_update draw the window's Philox
+   * kicks of neuron-trial idx itself, at rate
+   * drive_rate[min(t / drive_seg_steps, drive_n_seg - 1)][drive_group[idx]]
+   * with t the trial's own step -- the same bits fis_drive_table would
+   * set, without its launch.  Kicks already in sbits are kept.  */
+  const int16_t* drive_group;  /* [n * bstride], -1 = not table-driven */
+  const float* drive_rate;     /* [drive_n_seg][drive_n_groups] Hz     */
+  int32_t drive_n_groups;
+  int32_t drive_n_seg;
+  int32_t drive_seg_steps;
 } fis_sim;
 
 /* Drive table entry: neuron-trial `idx` is driven at rate

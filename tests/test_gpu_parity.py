@@ -134,10 +134,16 @@ def test_small10k_config1(golden):
     np.testing.assert_array_equal(r[0], np.minimum(z["refrac_final"][0], 127))
 
 
-@pytest.mark.parametrize("batch,road", [(3, "push"), (64, "push"),
-                                        (130, "push"), (64, "pull"),
-                                        (130, "pull"), (256, "pull")])
-def test_philox_drive_batched(batch, road):
+@pytest.mark.parametrize("batch,road,fused", [(3, "push", False),
+                                              (64, "push", False),
+                                              (130, "push", False),
+                                              (64, "pull", False),
+                                              (130, "pull", False),
+                                              (256, "pull", False),
+                                              (3, "push", True),
+                                              (64, "push", True),
+                                              (130, "push", True)])
+def test_philox_drive_batched(batch, road, fused):
     """Dense-rate Philox drive (Engine.step road) and per-trial drive tables
     against the oracle running the same Philox stream, trial by trial."""
     conn = C.synthetic(512, None, mean_out=40, count_exp=1.2, n_groups=4,
@@ -153,13 +159,15 @@ def test_philox_drive_batched(batch, road):
     tr = np.concatenate([np.full(len(d), b) for b, d in enumerate(drv)])
     sim.exempt_pairs(ne, tr)  # refractory exemption per trial
     table = DriveTable(sim, ne, tr, tr, np.array(rate, np.float32))
+    if fused:  # the update launches draw the table's kicks themselves
+        sim.fuse_drive(table)
     ora = O.OracleEngine(conn, batch, seeds=seeds, no_refractory_trials=drv)
     rates = np.zeros((batch, n), np.float32)
     for b, d in enumerate(drv):
         rates[b, d] = rate[b]
     steps = 400
     for a in range(0, steps, 20):
-        sim.window(20, table)
+        sim.window(20, None if fused else table)
         raster = sim.read_raster(a, a + 20)
         for k in range(20):
             ora.step_rates(rates)

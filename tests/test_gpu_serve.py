@@ -35,16 +35,20 @@ def net():
     return DeviceNet(conn)
 
 
-@pytest.mark.parametrize("road,slots,count", [("push", 6, 24),
-                                              ("pull", 6, 24),
-                                              ("pull", 70, 24),
-                                              ("pull", 128, 150)])
-def test_result_is_the_same_alone_and_in_the_mix(net, road, slots, count):
-    """More requests than slots: slots (and on the pull road whole 64-slot
-    words, skipped while idle) are freed and reused mid-run."""
+@pytest.mark.parametrize("road,slots,count,shard", [("push", 6, 24, 64),
+                                                    ("pull", 6, 24, 64),
+                                                    ("pull", 70, 24, 64),
+                                                    ("pull", 128, 150, 64),
+                                                    ("push", 40, 60, 16)])
+def test_result_is_the_same_alone_and_in_the_mix(net, road, slots, count,
+                                                 shard):
+    """More requests than slots: slots (and whole shards, launched only
+    while they hold a live request) are freed and reused mid-run."""
     reqs = requests(net.n, count, seed=slots)
-    mix, stats = Server(net, slots=slots, road=road).serve(
-        [SimRequest(**vars(r)) for r in reqs], realtime=False)
+    srv = Server(net, slots=slots, road=road, shard=shard)
+    assert len(srv.shards) == -(-slots // min(shard, slots))
+    mix, stats = srv.serve([SimRequest(**vars(r)) for r in reqs],
+                           realtime=False)
     assert stats["requests"] == len(reqs)
     got = {r.rid: r for r in mix}
     assert sorted(got) == [r.rid for r in reqs]
