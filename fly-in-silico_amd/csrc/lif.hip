@@ -21,6 +21,7 @@
 //    the same order (build with -ffp-contract=off).
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstddef>
 
 #include "../../include/fis.h"
@@ -240,12 +241,16 @@ __device__ __forceinline__ void emit(const fis_sim& s, bool live, bool fired,
   }
 }
 
-// Registers of one neuron-trial over a window.  The masks are the
-// reference's float factors (exactly 0 or 1, sim.py:393-410).
+// Registers of one neuron-trial over a window.  The reference's mask
+// factors (exactly 0.0 or 1.0, sim.py:393-410) become lane flags here; see
+// lif_step for why that leaves every value unchanged.
 struct Cell {
-  float v, g, sp;
-  int r, R;
-  float m_fire, m_in, m_hold, v_hold;
+  float v, g;
+  int r;     // refractory counter (saturating at FIS_ST_RMAX)
+  int R;     // gate threshold; INT_MAX when the input mask is off
+  bool sp;   // spiked at the previous step
+  bool fire_on, held;
+  float v_hold;
 };
 
 __device__ __forceinline__ void load_cell(const fis_sim& s, int idx, float& v,
@@ -286,61 +291,71 @@ __device__ __forceinline__ Cell make_cell(const fis_sim& s, uint32_t fl,
   Cell c;
   c.v = v;
   c.g = g;
-  c.m_fire = (fl & FIS_FIRE_OFF) ? 0.0f : 1.0f;
-  c.m_in = (fl & FIS_IN_OFF) ? 0.0f : 1.0f;
-  c.m_hold = (fl & FIS_HOLD) ? 0.0f : 1.0f;
-  c.v_hold = (fl & FIS_HOLD) ? vh : 0.0f;
-  c.R = (fl & FIS_NOREFRAC) ? 0 : s.refrac_steps;
+  c.fire_on = !(fl & FIS_FIRE_OFF);
+  c.held = (fl & FIS_HOLD) != 0;
+  c.v_hold = vh;
+  c.R = (fl & FIS_IN_OFF) ? INT_MAX
+                          : ((fl & FIS_NOREFRAC) ? 0 : s.refrac_steps);
   c.r = (int)(st & FIS_ST_RMAX);
-  c.sp = (st & FIS_ST_SPIKE) ? 1.0f : 0.0f;
+  c.sp = (st & FIS_ST_SPIKE) != 0;
   return c;
+}
+
+// The two values the reference's stimulus term can take,
+// w_scale * (bern * gain) with bern 1.0 or 0.0 (sim.py:859-861), each
+// computed with the reference's two roundings.
+struct Stim {
+  float on, off;
+};
+
+__device__ __forceinline__ Stim make_stim(const fis_sim& s) {
+  return Stim{s.w_scale * (1.0f * s.gain), s.w_scale * (0.0f * s.gain)};
 }
 
 // One step of Engine._core for one neuron-trial (sim.py:920-953), given the
 // exact integer synaptic input I read from the delay line and the Poisson
-// kick of the step.  Every line is one float32 rounding in the reference's
-// order.  Returns the masked spike (0 or 1).
-__device__ __forceinline__ float lif_step(const fis_sim& s, Cell& c, int32_t I,
-                                          uint32_t kick) {
-  // refractory counter: r = (r + 1) * (1 - s), saturating
-  c.r = (c.sp != 0.0f) ? 0 : (c.r < FIS_ST_RMAX ? c.r + 1 : FIS_ST_RMAX);
-  const float gate = (c.r >= c.R) ? 1.0f : 0.0f;
+// kick of the step.  Every operation whose operands are not a 0/1 mask is
+// one float32 rounding in the reference's order.  A product with a mask
+// factor that is exactly 1.0 or 0.0, and the sum of a finite x with a zero,
+// are x, x or a zero: the reference's
+//   delayed * gate, * m_in;  v * m_hold + v_hold;  spikes * m_fire;
+//   v - (v - v_reset) * spikes;  g_new - g_new * spikes
+// are therefore selects here, equal in value (a zero may differ in sign,
+// which no later +, -, * or comparison can turn into a different number).
+// Returns whether the neuron-trial fired (the masked spike).
+__device__ __forceinline__ bool lif_step(const fis_sim& s, Cell& c, int32_t I,
+                                         bool kick, const Stim& stim) {
+  // refractory counter: r = (r + 1) * (1 - s), saturating; gate = r >= R
+  c.r = c.sp ? 0 : (c.r < FIS_ST_RMAX ? c.r + 1 : FIS_ST_RMAX);
 
-  // conductance with the delayed input (sim.py:927-929)
+  // conductance with the delayed input (sim.py:927-929):
+  // g * a_syn + (w_scale * I) * m_in * gate
   const float rec = s.w_scale * (float)I;
-  const float del = rec * c.m_in;
   const float gd = c.g * s.a_syn;
-  const float dg = del * gate;
-  const float g_new = gd + dg;
+  const float g_new = (c.r >= c.R) ? gd + rec : gd;
 
-  // stim = w_scale * (bern * gain), sim.py:859-861
-  const float stim = s.w_scale * ((kick ? 1.0f : 0.0f) * s.gain);
   // membrane with the OLD conductance (sim.py:937-939)
-  float vv = c.v + stim;
+  float vv = c.v + (kick ? stim.on : stim.off);
   const float t1 = vv - s.v_rest;
   const float t2 = c.g - t1;
   const float t3 = s.a_mem * t2;
   vv = vv + t3;
-  vv = vv * c.m_hold;  // hold before threshold (sim.py:944)
-  vv = vv + c.v_hold;
-  float spk = (vv > s.v_th) ? 1.0f : 0.0f;
-  spk = spk * c.m_fire;
-  const float t4 = vv - s.v_reset;  // reset by the masked spike
-  const float t5 = t4 * spk;
-  vv = vv - t5;
-  const float gs = g_new * spk;
-  c.g = g_new - gs;
-  c.v = vv;
-  c.sp = spk;
-  return spk;
+  vv = c.held ? c.v_hold : vv;  // hold before threshold (sim.py:944)
+  const bool fire = c.fire_on && vv > s.v_th;
+  // reset by the masked spike
+  const float t4 = vv - s.v_reset;
+  c.v = fire ? vv - t4 : vv;
+  c.g = fire ? 0.0f : g_new;
+  c.sp = fire;
+  return fire;
 }
 
 __device__ __forceinline__ void store_cell(const fis_sim& s, const Cell& c,
                                            int idx, int pre, uint32_t nspk) {
   s.v[idx] = c.v;
   s.g[idx] = c.g;
-  s.st[idx] = (uint8_t)((c.sp != 0.0f ? FIS_ST_SPIKE : 0u) | (uint32_t)c.r);
-  if (s.spikes_out) s.spikes_out[idx] = c.sp;
+  s.st[idx] = (uint8_t)((c.sp ? FIS_ST_SPIKE : 0u) | (uint32_t)c.r);
+  if (s.spikes_out) s.spikes_out[idx] = c.sp ? 1.0f : 0.0f;
   if (nspk) {
     if (s.counts) s.counts[idx] += nspk;
     if (s.pop_counts) atomicAdd(&s.pop_counts[pre], nspk);
@@ -368,6 +383,7 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
   if (act && s.drive_group) sb |= fused_kicks(s, idx, pre, tb, t0, W);
 
   Cell c = make_cell(s, fl, vh, st, v, g);
+  const Stim stim = make_stim(s);
   const bool transmit = act && !(fl & FIS_OUT_OFF);
   const int64_t t_stop = act ? end_of(s, tb) : 0;
 
@@ -401,8 +417,7 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
     }
     if (act && I != 0) rp[(int64_t)slot * nb] = 0;
 
-    const float spk = lif_step(s, c, I, (sb >> k) & 1u);
-    const bool fired = act && spk != 0.0f;
+    const bool fired = lif_step(s, c, I, (sb >> k) & 1u, stim) && act;
     nspk += (fired && t0 + k < t_stop) ? 1u : 0u;
     emit(s, act, fired, fired && transmit, lane, pre, tb, wave, nblk, rrow,
          slot, k, evs, nev);
@@ -523,27 +538,29 @@ __global__ __launch_bounds__(kBlock) void k_resend(fis_sim s,
 
 // ------------------------------------------------------------- pull road
 //
-// One launch per window.  Workgroup = max(FIS_PULL_TILE, bstride)
-// neuron-trials = P neurons x all bstride trials (bstride a power of two,
-// 64..1024), so the tile is a contiguous idx range.  Records are laid out [pre][slot][word]:
-// one neuron's 40 steps x bstride bits (320 B at 64 trials) belong to the
-// workgroup that owns the neuron, and the 20 steps of one delay horizon
-// sit in a few lines for the gather.
+// One launch per window.  A workgroup ("tile") is TILE neuron-trials: P =
+// TILE / bstride whole neurons when TILE >= bstride, else one neuron's
+// trials split into bstride / TILE parts (bstride a power of two, 64..1024).
+// Records are laid out [pre][slot][word]: one neuron's 40 steps x bstride
+// bits (320 B at 64 trials) belong to the workgroups that own the neuron,
+// and the 20 steps of one delay horizon sit in a few lines for the gather.
 //
-//  1. gather: the tile's in-edges (W CSR by post) are walked once; for each
-//     in-edge whose presynaptic neuron was active at some step t0-20+j of
-//     the window's delay horizon (act word), the 64-trial record words of
-//     that step are read and the synapse count is added into LDS
-//     acc[j][post][trial] for every set trial bit -- exact int32, order-free;
+//  1. gather: the tile's in-edges (W CSR by post) are walked once per
+//     64-trial word; for each in-edge whose presynaptic neuron had a spike
+//     in the tile's trials at some step t0-20+j of the window's delay
+//     horizon (act word of the neuron and part), the record word of that
+//     step is read and the synapse count is added into LDS
+//     acc[j][post][trial] for every set trial bit -- exact int32,
+//     order-free;
 //  2. update: W steps of lif_step per neuron-trial with I = acc[j][...],
-//     state in registers, spike records of steps t0..t0+W-1 written;
-//  3. the tile's act words get the window's bits (device atomics: the act
-//     words of several tiles share lines).
+//     state in registers; each wave (one neuron x 64 trials) writes its
+//     spike record words of steps t0..t0+W-1;
+//  3. the tile's act words (one per neuron and part) get the window's bits
+//     (device atomics: the act words of several tiles share lines).
 // Records and act bits of steps t0-20.. (read) and t0.. (written) are
 // disjoint slots mod 40 for W <= 20, so no workgroup reads what another
 // writes in the same launch.
 
-constexpr int kPullTile = FIS_PULL_TILE;
 constexpr int kRec = FIS_REC_SLOTS;
 constexpr uint64_t kMask40 = (1ull << kRec) - 1;
 
@@ -553,23 +570,46 @@ __device__ __forceinline__ uint64_t rot40(uint64_t x, int r) {
   return r == 0 ? x : (((x << r) | (x >> (kRec - r))) & kMask40);
 }
 
-template <int BS>
-constexpr int pull_tile() {
-  return BS > kPullTile ? BS : kPullTile;
+// acc[b] += val for every set bit b of the 64-trial word x
+__device__ __forceinline__ void add_bits(int32_t* acc, uint64_t x,
+                                         int32_t val) {
+  while (x) {
+    const int b = __ffsll((long long)x) - 1;
+    x &= x - 1ull;
+    atomicAdd(acc + b, val);
+  }
 }
 
-template <int WMAX, int BS>
-__global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
-                                                          int ring_steps,
-                                                          int n_tiles,
-                                                          int per_xcd) {
-  constexpr int kPull = pull_tile<BS>();  // neuron-trials per workgroup
-  // WMAX x 2 KB of LDS and nothing else: several workgroups per CU
-  __shared__ int32_t acc[WMAX * kPull];
+// Neuron-trials per workgroup for a trial stride.  256 lanes (4 waves,
+// 20 KB of accumulators) measured fastest at 256 trials: smaller
+// workgroups wait less at the barrier between gather and update for their
+// slowest wave, and 8 of them fit a CU.
+template <int BS>
+constexpr int pull_tile() {
+  return FIS_PULL_TILE;
+}
 
-  constexpr int bs = BS;           // trial stride (== s.bstride)
-  constexpr int P = kPull / BS;    // neurons per tile
-  constexpr int words = BS >> 6;   // 64-trial words per neuron
+// trials of one neuron in one tile, and tiles per neuron (act words per
+// neuron), for a trial stride of the pull road
+__host__ __device__ constexpr int pull_trials_per_part(int bs) {
+  return bs < FIS_PULL_TILE ? bs : FIS_PULL_TILE;
+}
+__host__ __device__ constexpr int pull_parts(int bs) {
+  return bs / pull_trials_per_part(bs);
+}
+
+template <int WMAX, int BS, int TILE>
+__global__ __launch_bounds__(TILE) void k_pull(fis_sim s, int W,
+                                               int ring_steps, int n_tiles,
+                                               int per_xcd) {
+  __shared__ int32_t acc[WMAX * TILE];  // WMAX x 4 B per lane, nothing else
+
+  constexpr int bs = BS;                        // trial stride (== s.bstride)
+  constexpr int tpn = TILE < BS ? TILE : BS;    // trials of a neuron per tile
+  constexpr int P = TILE / tpn;                 // neurons per tile
+  constexpr int parts = BS / tpn;               // tiles per neuron
+  constexpr int words = BS >> 6;                // 64-trial words per neuron
+  constexpr int wpt = tpn >> 6;                 // of them in one tile
   const int tid = (int)threadIdx.x;
   // XCD-aware tile order: workgroup b is dispatched to XCD b mod 8, and
   // XCD x takes the consecutive tiles [x * per_xcd, (x + 1) * per_xcd) of
@@ -577,7 +617,9 @@ __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
   // neurons' presynaptic partners share
   const int tile = (int)(blockIdx.x & 7u) * per_xcd + (int)(blockIdx.x >> 3);
   if (tile >= n_tiles) return;  // whole workgroup, before any barrier
-  const int p0 = tile * P;      // first position of the tile
+  const int p0 = (tile / parts) * P;   // first position of the tile
+  const int part = tile % parts;       // which part of its neuron(s)
+  const int w0 = part * wpt;           // first word of the tile
   const int np = min(P, s.n - p0);
   const int64_t t0 = *s.clock;
   const int rbase = (int)((t0 + kRec - 20) % kRec);  // slot of step t0-20
@@ -585,7 +627,7 @@ __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
   const int pstride = kRec * words;                  // words per neuron
   const uint64_t wmask = (1ull << W) - 1;
 
-  for (int j = 0; j < W; ++j) acc[j * kPull + tid] = 0;
+  for (int j = 0; j < W; ++j) acc[j * TILE + tid] = 0;
   // the tile's in-edge offsets: uniform, held in scalar registers
   int off[P + 1];
 #pragma unroll
@@ -594,14 +636,15 @@ __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
   __syncthreads();
 
   {  // 1. gather; the next in-edge is fetched while this one is summed,
-     // and the record loads of up to 8 steps are in flight per lane
+     // and the record loads of up to 10 steps are in flight per lane
     constexpr int kIn = 8;
-    const int w = tid & (words - 1);
-    const int G = kPull / words;
+    const int wl = tid & (wpt - 1);  // word of this lane within the tile
+    const int w = w0 + wl;
+    constexpr int G = TILE / wpt;    // in-edges per workgroup iteration
     const bool wlive = !s.word_end || t0 < s.word_end[w];
     const bool nowrap = rbase + W <= kRec;  // uniform
     const int e1 = s.in_offsets[p0 + np];
-    int e = off[0] + tid / words;
+    int e = off[0] + tid / wpt;
     int pre = 0;
     int32_t val = 0;
     if (e < e1) {
@@ -620,8 +663,9 @@ __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
 #pragma unroll
       for (int q = 1; q < P; ++q) pl += e >= off[q] ? 1 : 0;
       uint64_t m =
-          wlive ? rot40(s.act[pre], (kRec - rbase) % kRec) & wmask : 0ull;
-      int32_t* ap = acc + pl * bs + (w << 6);
+          wlive ? rot40(s.act[pre * parts + part], (kRec - rbase) % kRec) & wmask
+                : 0ull;
+      int32_t* ap = acc + pl * tpn + (wl << 6);
       const uint64_t* rp = s.rec + (int64_t)pre * pstride + w;
       if (nowrap) {
         // the horizon's slots rbase .. rbase+W-1 are consecutive: step j's
@@ -637,15 +681,7 @@ __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
             x[q] = (h + q < WMAX && ((m32 >> (h + q)) & 1u))
                        ? rq[(h + q) * words] : 0ull;
 #pragma unroll
-          for (int q = 0; q < 10; ++q) {
-            uint64_t bits = x[q];
-            int32_t* aj = ap + (h + q) * kPull;
-            while (bits) {
-              const int b = __ffsll((long long)bits) - 1;
-              bits &= bits - 1ull;
-              atomicAdd(aj + b, val);
-            }
-          }
+          for (int q = 0; q < 10; ++q) add_bits(ap + (h + q) * TILE, x[q], val);
         }
         m = 0;
       }
@@ -664,14 +700,7 @@ __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
           x[q] = j[q] >= 0 ? rp[sl * words] : 0ull;
         }
 #pragma unroll
-        for (int q = 0; q < kIn; ++q) {
-          uint64_t bits = x[q];
-          while (bits) {
-            const int b = __ffsll((long long)bits) - 1;
-            bits &= bits - 1ull;
-            atomicAdd(ap + j[q] * kPull + b, val);
-          }
-        }
+        for (int q = 0; q < kIn; ++q) add_bits(ap + j[q] * TILE, x[q], val);
       }
       e = en;
       pre = pre_n;
@@ -683,8 +712,8 @@ __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
   // 2. update
   const int nb = s.n * bs;
   const int lane = tid & (kWave - 1);
-  const int pl = tid / bs;           // position within the tile
-  const int tb = tid - pl * bs;      // trial of this lane
+  const int pl = tid / tpn;                          // position in the tile
+  const int tb = (w0 << 6) + (tid - pl * tpn);       // trial of this lane
   // uniform over a wave: a neuron of the tile, in a live 64-trial word
   const bool act = pl < np && (!s.word_end || t0 < s.word_end[tb >> 6]);
   const int pre = pl < np ? s.order[p0 + pl] : 0;  // neuron of this lane
@@ -696,6 +725,7 @@ __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
   uint32_t st = 0, fl = FIS_FIRE_OFF, sb = 0;
   if (act) load_cell(s, idx, v, g, st, fl, sb, vh);
   Cell c = make_cell(s, fl, vh, st, v, g);
+  const Stim stim = make_stim(s);
   const bool transmit = act && !(fl & FIS_OUT_OFF);
   // steps of this window whose spikes still count for the trial
   const int64_t left = act ? end_of(s, tb) - t0 : 0;
@@ -712,7 +742,7 @@ __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
 
 #pragma unroll 1
   for (int k = 0; k < W; ++k) {
-    int32_t* row = acc + k * kPull;
+    int32_t* row = acc + k * TILE;
     int32_t I = row[tid];
     // the wave's 64 cells of this row are free once read: two of them
     // collect the wave's spike words (LDS atomics of the spiking lanes)
@@ -729,8 +759,7 @@ __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
         rp[(int64_t)rslot * nb] = 0;
       }
     }
-    const float spk = lif_step(s, c, I, (sb >> k) & 1u);
-    const bool fired = act && spk != 0.0f;
+    const bool fired = lif_step(s, c, I, (sb >> k) & 1u, stim) && act;
     nspk += (fired && k < kstop) ? 1u : 0u;
     const bool sends = fired && transmit;
     if (sends) atomicOr(&scr[0], 1ull << lane);
@@ -762,8 +791,8 @@ __global__ __launch_bounds__(pull_tile<BS>()) void k_pull(fis_sim s, int W,
   // 3. act bits of the window's steps: clear, then set
   if (tid < np) {
     uint32_t bits32 = 0;
-    for (int q = 0; q < words; ++q) bits32 |= (uint32_t)acc[tid * bs + q * kWave];
-    uint64_t* a = s.act + s.order[p0 + tid];
+    for (int q = 0; q < wpt; ++q) bits32 |= (uint32_t)acc[tid * tpn + q * kWave];
+    uint64_t* a = s.act + (int64_t)s.order[p0 + tid] * parts + part;
     __hip_atomic_fetch_and(a, ~rot40(wmask, wbase), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     const uint64_t bits = rot40((uint64_t)bits32, wbase);
@@ -798,7 +827,9 @@ __global__ __launch_bounds__(kBlock) void k_pull_resend(fis_sim s,
     __hip_atomic_fetch_and(w, ~bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   } else {
     __hip_atomic_fetch_or(w, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_or(s.act + pre, 1ull << slot, __ATOMIC_RELAXED,
+    __hip_atomic_fetch_or(s.act + (int64_t)pre * pull_parts(s.bstride) +
+                              b / pull_trials_per_part(s.bstride),
+                          1ull << slot, __ATOMIC_RELAXED,
                           __HIP_MEMORY_SCOPE_AGENT);
   }
 }
@@ -814,11 +845,13 @@ __global__ __launch_bounds__(kBlock) void k_pull_to_ring(fis_sim s,
   const int64_t u = *s.clock - 20 + (int64_t)blockIdx.y;
   if (u < 0) return;
   const int slot = (int)(u % kRec);
-  if (!((s.act[pre] >> slot) & 1ull)) return;
   const int words = s.bstride >> 6;
   const int64_t nb = (int64_t)s.n * s.bstride;
   int32_t* base = ring + (int64_t)((u + 1) % FIS_RING_LEN) * nb;
+  const int parts = pull_parts(s.bstride);
+  const int wpp = words / parts;  // words per part
   for (int w = 0; w < words; ++w) {
+    if (!((s.act[(int64_t)pre * parts + w / wpp] >> slot) & 1ull)) continue;
     const uint64_t m = s.rec[((int64_t)pre * kRec + slot) * words + w];
     if (m == 0) continue;
     for (int64_t k = s.offsets[pre] + lane; k < s.offsets[pre + 1];
@@ -1108,21 +1141,29 @@ int fis_pull_window(const fis_sim* s, int W, int ring_steps, void* stream) {
   if (int rc = check_pull(s)) return rc;
   if (W < 1 || W > FIS_MAX_WINDOW || ring_steps < 0)
     return (int)hipErrorInvalidValue;
-  const int64_t nb = (int64_t)s->n * s->bstride;
-  auto kern = k_pull<FIS_MAX_WINDOW, 256>;
+  auto kern = k_pull<FIS_MAX_WINDOW, 256, pull_tile<256>()>;
   int tile = 0;
   switch (s->bstride) {
-    case 64: kern = k_pull<FIS_MAX_WINDOW, 64>; tile = pull_tile<64>(); break;
-    case 128: kern = k_pull<FIS_MAX_WINDOW, 128>; tile = pull_tile<128>(); break;
-    case 256: kern = k_pull<FIS_MAX_WINDOW, 256>; tile = pull_tile<256>(); break;
-    case 512: kern = k_pull<FIS_MAX_WINDOW, 512>; tile = pull_tile<512>(); break;
-    case 1024: kern = k_pull<FIS_MAX_WINDOW, 1024>; tile = pull_tile<1024>(); break;
+#define FIS_PULL_CASE(B)                                  \
+  case B:                                                 \
+    kern = k_pull<FIS_MAX_WINDOW, B, pull_tile<B>()>;     \
+    tile = pull_tile<B>();                                \
+    break;
+    FIS_PULL_CASE(64)
+    FIS_PULL_CASE(128)
+    FIS_PULL_CASE(256)
+    FIS_PULL_CASE(512)
+    FIS_PULL_CASE(1024)
+#undef FIS_PULL_CASE
     default: return (int)hipErrorInvalidValue;
   }
-  const int P = tile / s->bstride;
-  const int n_tiles = (s->n + P - 1) / P;
-  const int per_xcd = (n_tiles + 7) / 8;
-  (void)nb;
+  // tiles: P neurons each, or `parts` tiles per neuron
+  const int P = tile >= s->bstride ? tile / s->bstride : 1;
+  const int parts = tile >= s->bstride ? 1 : s->bstride / tile;
+  const int n_tiles = ((s->n + P - 1) / P) * parts;
+  // whole neurons per XCD: the parts of one neuron write the same record
+  // lines, and share one L2
+  const int per_xcd = ((n_tiles + 7) / 8 + parts - 1) / parts * parts;
   hipLaunchKernelGGL(kern, dim3((unsigned)(8 * per_xcd)), dim3(tile), 0,
                      (hipStream_t)stream, *s, W, ring_steps, n_tiles, per_xcd);
   hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, (hipStream_t)stream,

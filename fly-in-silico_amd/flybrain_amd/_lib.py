@@ -20,13 +20,13 @@ CSRC = PKG.parent / "csrc"
 SOURCES = [CSRC / "lif.hip"]
 HEADER = ROOT / "include" / "fis.h"
 LIB_PATH = PKG / "libfis.so"
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 MAX_WINDOW = 20
 RING_LEN = 20
 FIRE_OFF, OUT_OFF, IN_OFF, HOLD, NOREFRAC = 0x01, 0x02, 0x04, 0x08, 0x10
 BLOCK_WAVES, LINE_WORDS, LINE_INTS, EVENT_ALIGN = 4, 16, 32, 8
-REC_SLOTS, PULL_TILE, PULL_MAX = 40, 512, 1024
+REC_SLOTS, PULL_TILE, PULL_MAX = 40, 256, 1024
 ST_SPIKE, ST_RMAX = 0x80, 127
 
 HIPCC_FLAGS = [

@@ -207,7 +207,8 @@ class DeviceSim:
                 net.pull_layout(bs)
             self.rec = torch.zeros(L.REC_SLOTS * n * (bs // 64),
                                    dtype=torch.int64, **z)
-            self.act = torch.zeros(n, dtype=torch.int64, **z)
+            self.act = torch.zeros(n * max(1, bs // L.PULL_TILE),
+                                   dtype=torch.int64, **z)
         else:
             self.order = self.in_offsets = self.in_cols = self.in_vals = None
             self.rec = self.act = None

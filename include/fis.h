@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define FIS_ABI_VERSION 5
+#define FIS_ABI_VERSION 6
 #define FIS_MAX_WINDOW 20   /* steps per window: delay horizon 19 + 1     */
 #define FIS_RING_LEN 20     /* int32 ring slots: rec(t-19) .. rec(t)       */
 
@@ -48,8 +48,9 @@ extern "C" {
 #define FIS_LINE_INTS 32    /* 128 bytes = 32 x 4-byte ints               */
 #define FIS_EVENT_ALIGN 8   /* 8 events = 128 bytes                       */
 #define FIS_REC_SLOTS 40    /* pull road: spike records of steps t mod 40 */
-#define FIS_PULL_TILE 512   /* pull road: neuron-trials per workgroup
-                               (bstride 1024: one neuron, 1024)           */
+#define FIS_PULL_TILE 256   /* pull road: neuron-trials per workgroup:
+                               256 / bstride neurons, or a quarter (512:
+                               half) of one neuron's trials               */
 
 /* flag bits of `fl`, one byte per neuron-trial (sim.py:393-410, :866-900) */
 #define FIS_FIRE_OFF 0x01u  /* m_fire == 0: spike output suppressed        */
@@ -129,12 +130,13 @@ typedef struct {
    * neuron (W CSR, the reference's own W[post, pre] orientation) and the
    * spike records of the last 40 steps.  rec word
    * [pre][t mod 40][w] bit j = trial 64 w + j of `pre` transmitted a spike
-   * at step t; act[pre] bit (t mod 40) = some trial of `pre` did.         */
+   * at step t; act[pre][k] bit (t mod 40) = some trial of `pre` among
+   * trials k * FIS_PULL_TILE .. (k + 1) * FIS_PULL_TILE - 1 did.         */
   const int32_t* in_offsets; /* [n+1], by position (see `order`)       */
   const int32_t* in_cols;    /* [E] presynaptic neuron                 */
   const int16_t* in_vals;    /* [E] signed synapse count               */
   uint64_t* rec;             /* [n][40][bstride / 64]                  */
-  uint64_t* act;             /* [n]                                    */
+  uint64_t* act;             /* [n][max(1, bstride / FIS_PULL_TILE)]   */
   /* the order in which workgroups take the neurons: position k holds
    * neuron order[k] (a permutation of 0..n-1 that moves aligned units of
    * max(1, 256 / bstride) neurons, e.g. sorted by neuropil group, so that
