@@ -730,68 +730,86 @@ __global__ __launch_bounds__(TILE) void k_pull(fis_sim s, int W,
   // steps of this window whose spikes still count for the trial
   const int64_t left = act ? end_of(s, tb) - t0 : 0;
   const int kstop = left <= 0 ? 0 : (left >= W ? W : (int)left);
-  const bool use_ring = ring_steps > 0;  // uniform
 
-  uint64_t* wp = s.rec + (int64_t)pre * pstride + (tb >> 6);
-  int32_t* rp = s.ring + idx;
-  int rslot = (int)((t0 + 1) % FIS_RING_LEN);
-  int wslot = wbase;
-  uint32_t nspk = 0;
-  uint32_t abits = 0;  // wave-uniform: steps at which this wave sent
-  int rrow = s.raster ? (int)(t0 % s.raster_len) : 0;
-
-#pragma unroll 1
-  for (int k = 0; k < W; ++k) {
-    int32_t* row = acc + k * TILE;
-    int32_t I = row[tid];
-    // the wave's 64 cells of this row are free once read: two of them
-    // collect the wave's spike words (LDS atomics of the spiking lanes)
-    uint64_t* scr = (uint64_t*)(row + (tid & ~(kWave - 1)));
-    if (lane == 0) {
-      scr[0] = 0ull;
-      scr[1] = 0ull;
-    }
-    __builtin_amdgcn_wave_barrier();
-    if (use_ring && k < ring_steps && act) {  // input carried over from a
-      const int32_t x = rp[(int64_t)rslot * nb];  // restore
-      if (x != 0) {
-        I += x;
-        rp[(int64_t)rslot * nb] = 0;
+  // the window's inputs, all read before the recurrence starts (nothing in
+  // it waits for memory)
+  int32_t in[WMAX];
+#pragma unroll
+  for (int k = 0; k < WMAX; ++k) in[k] = k < W ? acc[k * TILE + tid] : 0;
+  if (ring_steps > 0 && act) {  // input carried over from a restore
+    int32_t* rp = s.ring + idx;
+    int rslot = (int)((t0 + 1) % FIS_RING_LEN);
+#pragma unroll
+    for (int k = 0; k < WMAX; ++k) {
+      if (k < W && k < ring_steps) {
+        const int32_t x = rp[(int64_t)rslot * nb];
+        if (x != 0) {
+          in[k] += x;
+          rp[(int64_t)rslot * nb] = 0;
+        }
       }
+      rslot = (rslot + 1 == FIS_RING_LEN) ? 0 : rslot + 1;
     }
-    const bool fired = lif_step(s, c, I, (sb >> k) & 1u, stim) && act;
-    nspk += (fired && k < kstop) ? 1u : 0u;
-    const bool sends = fired && transmit;
-    if (sends) atomicOr(&scr[0], 1ull << lane);
-    if (fired && s.raster) atomicOr(&scr[1], 1ull << lane);
-    __builtin_amdgcn_wave_barrier();
-    const uint64_t msk = scr[0];
-    // `act` is uniform over a wave (n * bstride is a multiple of 64): the
-    // waves past the last neuron of the last tile store nothing
-    if (lane == 0 && act) {
-      wp[wslot * words] = msk;
-      if (s.raster)
-        s.raster[((int64_t)rrow * nblk + (wave >> 2)) * FIS_LINE_WORDS +
-                 (wave & 3)] = scr[1];
+  }
+
+  uint32_t fired_bits = 0;  // bit k: this lane fired at step k
+  uint32_t nspk = 0;
+#pragma unroll
+  for (int k = 0; k < WMAX; ++k) {
+    if (k < W) {
+      const bool fired = lif_step(s, c, in[k], (sb >> k) & 1u, stim) && act;
+      nspk += (fired && k < kstop) ? 1u : 0u;
+      fired_bits |= fired ? (1u << k) : 0u;
     }
-    if (s.raster) rrow = (rrow + 1 == s.raster_len) ? 0 : rrow + 1;
-    // msk is the same in every lane: test it in scalar registers
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)msk);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(msk >> 32));
-    abits |= (lo | hi) != 0u ? (1u << k) : 0u;
-    rslot = (rslot + 1 == FIS_RING_LEN) ? 0 : rslot + 1;
-    wslot = (wslot + 1 == kRec) ? 0 : wslot + 1;
   }
   if (act) store_cell(s, c, idx, pre, nspk);
-  // a thread's own step-0 cell is free once read: the first lane of each
-  // wave parks its step bits there
-  if (lane == 0) acc[tid] = (int32_t)abits;
+
+  // The wave's record words of the window (and raster rows): lane k < W
+  // owns step k's.  This wave's 64 cells of every acc row are its own and
+  // already read; row k's first two 8-byte cells collect step k's words
+  // (LDS atomics of the lanes that fired, few), cell 4 of row 0 the steps
+  // at which the wave sent anything.
+  {
+    const uint32_t sent = transmit ? fired_bits : 0u;
+    int32_t* mine = acc + (tid & ~(kWave - 1));  // this wave's cells, row 0
+    uint64_t* cell = (uint64_t*)(mine + (lane < W ? lane : 0) * TILE);
+    if (lane < W) {
+      cell[0] = 0ull;
+      cell[1] = 0ull;
+    }
+    if (lane == W) mine[4] = 0;  // (row 0: its words sit in cells 0..3)
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t me = 1ull << lane;
+    for (uint32_t x = sent; x; x &= x - 1u)
+      atomicOr((uint64_t*)(mine + (__ffs(x) - 1) * TILE), me);
+    if (s.raster)
+      for (uint32_t x = fired_bits; x; x &= x - 1u)
+        atomicOr((uint64_t*)(mine + (__ffs(x) - 1) * TILE) + 1, me);
+    __builtin_amdgcn_wave_barrier();
+    if (lane < W) {
+      const uint64_t word = cell[0];
+      if (word != 0ull) atomicOr(mine + 4, 1 << lane);
+      // `act` is uniform over a wave (n * bstride is a multiple of 64): the
+      // waves past the last neuron of the last tile store nothing
+      if (act) {
+        const int slot = wbase + lane < kRec ? wbase + lane : wbase + lane - kRec;
+        s.rec[(int64_t)pre * pstride + slot * words + (tb >> 6)] = word;
+        if (s.raster) {
+          int rrow = (int)(t0 % s.raster_len) + lane;  // raster_len >= 20
+          rrow = rrow >= s.raster_len ? rrow - s.raster_len : rrow;
+          s.raster[((int64_t)rrow * nblk + (wave >> 2)) * FIS_LINE_WORDS +
+                   (wave & 3)] = cell[1];
+        }
+      }
+    }
+  }
   __syncthreads();
 
   // 3. act bits of the window's steps: clear, then set
   if (tid < np) {
     uint32_t bits32 = 0;
-    for (int q = 0; q < wpt; ++q) bits32 |= (uint32_t)acc[tid * tpn + q * kWave];
+    for (int q = 0; q < wpt; ++q)
+      bits32 |= (uint32_t)acc[tid * tpn + q * kWave + 4];
     uint64_t* a = s.act + (int64_t)s.order[p0 + tid] * parts + part;
     __hip_atomic_fetch_and(a, ~rot40(wmask, wbase), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
