@@ -387,42 +387,53 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
   const bool transmit = act && !(fl & FIS_OUT_OFF);
   const int64_t t_stop = act ? end_of(s, tb) : 0;
 
-  // ring reads run PREFETCH steps ahead of the recurrence (they do not
-  // depend on it); the step loop itself stays rolled, which keeps the
-  // per-step state out of scalar registers
-  constexpr int PREFETCH = WMAX < 4 ? WMAX : 4;
+  // the window's ring slots, all read (and the non-zero ones cleared)
+  // before the recurrence starts: nothing in it waits for memory
   int32_t* rp = s.ring + idx;
-  int slot = (int)((t0 + 1) % FIS_RING_LEN);
-  int32_t q[PREFETCH];
-  int pslot = slot;
+  const int slot0 = (int)((t0 + 1) % FIS_RING_LEN);
+  int32_t in[WMAX];
+  {
+    int sl = slot0;
 #pragma unroll
-  for (int j = 0; j < PREFETCH; ++j) {
-    q[j] = (j < W && act) ? rp[(int64_t)pslot * nb] : 0;
-    pslot = (pslot + 1 == FIS_RING_LEN) ? 0 : pslot + 1;
+    for (int k = 0; k < WMAX; ++k) {
+      in[k] = (k < W && act) ? rp[(int64_t)sl * nb] : 0;
+      sl = (sl + 1 == FIS_RING_LEN) ? 0 : sl + 1;
+    }
+    sl = slot0;
+#pragma unroll
+    for (int k = 0; k < WMAX; ++k) {
+      if (in[k] != 0) rp[(int64_t)sl * nb] = 0;
+      sl = (sl + 1 == FIS_RING_LEN) ? 0 : sl + 1;
+    }
   }
 
+  uint32_t fired_bits = 0;  // bit k: this lane fired at step k
+  uint32_t nspk = 0;
+#pragma unroll
+  for (int k = 0; k < WMAX; ++k) {
+    if (k < W) {
+      const bool fired = lif_step(s, c, in[k], (sb >> k) & 1u, stim) && act;
+      nspk += (fired && t0 + k < t_stop) ? 1u : 0u;
+      fired_bits |= fired ? (1u << k) : 0u;
+    }
+  }
+
+  // spikes out, step by step: raster rows, and events (below 64 trials
+  // only steps with spikes append any, so a wave that stayed silent for the
+  // whole window skips this)
   fis_event* evs = s.events + (int64_t)wave * s.ev_cap;
   int nev = 0;  // wave-uniform
-  uint32_t nspk = 0;
-  int rrow = s.raster ? (int)(t0 % s.raster_len) : 0;
-
+  if (s.raster || bs >= kWave || any_lane(fired_bits != 0u)) {
+    int rrow = s.raster ? (int)(t0 % s.raster_len) : 0;
+    int slot = slot0;
 #pragma unroll 1
-  for (int k = 0; k < W; ++k) {
-    const int32_t I = q[0];
-#pragma unroll
-    for (int j = 0; j + 1 < PREFETCH; ++j) q[j] = q[j + 1];
-    if (PREFETCH > 0) {
-      q[PREFETCH - 1] = (k + PREFETCH < W && act) ? rp[(int64_t)pslot * nb] : 0;
-      pslot = (pslot + 1 == FIS_RING_LEN) ? 0 : pslot + 1;
+    for (int k = 0; k < W; ++k) {
+      const bool fired = (fired_bits >> k) & 1u;
+      emit(s, act, fired, fired && transmit, lane, pre, tb, wave, nblk, rrow,
+           slot, k, evs, nev);
+      rrow = (rrow + 1 == s.raster_len) ? 0 : rrow + 1;
+      slot = (slot + 1 == FIS_RING_LEN) ? 0 : slot + 1;
     }
-    if (act && I != 0) rp[(int64_t)slot * nb] = 0;
-
-    const bool fired = lif_step(s, c, I, (sb >> k) & 1u, stim) && act;
-    nspk += (fired && t0 + k < t_stop) ? 1u : 0u;
-    emit(s, act, fired, fired && transmit, lane, pre, tb, wave, nblk, rrow,
-         slot, k, evs, nev);
-    rrow = (rrow + 1 == s.raster_len) ? 0 : rrow + 1;
-    slot = (slot + 1 == FIS_RING_LEN) ? 0 : slot + 1;
   }
 
   if (act) store_cell(s, c, idx, pre, nspk);
