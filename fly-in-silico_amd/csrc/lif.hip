@@ -609,8 +609,14 @@ __host__ __device__ constexpr int pull_parts(int bs) {
   return bs / pull_trials_per_part(bs);
 }
 
+// Scalar registers capped at 80: measured on MI355X, a workgroup of this
+// kernel is resident one fewer per CU than the compiler's occupancy
+// estimate whenever the kernel uses more than ~80 (96 SGPRs: 7 of 8
+// workgroups, 1.465 ms per window; capped at 80: 8, 1.417 ms).
 template <int WMAX, int BS, int TILE>
-__global__ __launch_bounds__(TILE) void k_pull(fis_sim s, int W,
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_num_sgpr(80)))
+__attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_pull(fis_sim s, int W,
                                                int ring_steps, int n_tiles,
                                                int per_xcd) {
   __shared__ int32_t acc[WMAX * TILE];  // WMAX x 4 B per lane, nothing else
@@ -646,81 +652,8 @@ __global__ __launch_bounds__(TILE) void k_pull(fis_sim s, int W,
     off[q] = q <= np ? s.in_offsets[p0 + q] : 0x7fffffff;
   __syncthreads();
 
-  {  // 1. gather; the next in-edge is fetched while this one is summed,
-     // and the record loads of up to 10 steps are in flight per lane
-    constexpr int kIn = 8;
-    const int wl = tid & (wpt - 1);  // word of this lane within the tile
-    const int w = w0 + wl;
-    constexpr int G = TILE / wpt;    // in-edges per workgroup iteration
-    const bool wlive = !s.word_end || t0 < s.word_end[w];
-    const bool nowrap = rbase + W <= kRec;  // uniform
-    const int e1 = s.in_offsets[p0 + np];
-    int e = off[0] + tid / wpt;
-    int pre = 0;
-    int32_t val = 0;
-    if (e < e1) {
-      pre = s.in_cols[e];
-      val = s.in_vals[e];
-    }
-    while (e < e1) {
-      const int en = e + G;
-      int pre_n = 0;
-      int32_t val_n = 0;
-      if (en < e1) {
-        pre_n = s.in_cols[en];
-        val_n = s.in_vals[en];
-      }
-      int pl = 0;
-#pragma unroll
-      for (int q = 1; q < P; ++q) pl += e >= off[q] ? 1 : 0;
-      uint64_t m =
-          wlive ? rot40(s.act[pre * parts + part], (kRec - rbase) % kRec) & wmask
-                : 0ull;
-      int32_t* ap = acc + pl * tpn + (wl << 6);
-      const uint64_t* rp = s.rec + (int64_t)pre * pstride + w;
-      if (nowrap) {
-        // the horizon's slots rbase .. rbase+W-1 are consecutive: step j's
-        // word sits at a fixed offset from one base (no per-step slot and
-        // address arithmetic); two rounds of 10 predicated loads
-        const uint64_t* rq = rp + rbase * words;
-        const uint32_t m32 = (uint32_t)m;
-#pragma unroll
-        for (int h = 0; h < WMAX; h += 10) {
-          uint64_t x[10];
-#pragma unroll
-          for (int q = 0; q < 10; ++q)
-            x[q] = (h + q < WMAX && ((m32 >> (h + q)) & 1u))
-                       ? rq[(h + q) * words] : 0ull;
-#pragma unroll
-          for (int q = 0; q < 10; ++q) add_bits(ap + (h + q) * TILE, x[q], val);
-        }
-        m = 0;
-      }
-      while (m) {
-        int j[kIn];
-        uint64_t x[kIn];
-#pragma unroll
-        for (int q = 0; q < kIn; ++q) {
-          j[q] = m ? __ffsll((long long)m) - 1 : -1;
-          m &= m - 1ull;
-        }
-#pragma unroll
-        for (int q = 0; q < kIn; ++q) {
-          int sl = rbase + j[q];
-          sl = sl >= kRec ? sl - kRec : sl;
-          x[q] = j[q] >= 0 ? rp[sl * words] : 0ull;
-        }
-#pragma unroll
-        for (int q = 0; q < kIn; ++q) add_bits(ap + j[q] * TILE, x[q], val);
-      }
-      e = en;
-      pre = pre_n;
-      val = val_n;
-    }
-  }
-  __syncthreads();
-
-  // 2. update
+  // the update's neuron-trial: its state is read now, so that the loads
+  // overlap the gather
   const int nb = s.n * bs;
   const int lane = tid & (kWave - 1);
   const int pl = tid / tpn;                          // position in the tile
@@ -731,10 +664,93 @@ __global__ __launch_bounds__(TILE) void k_pull(fis_sim s, int W,
   const int idx = pre * bs + tb;
   const int wave = idx >> 6;
   const int nblk = (nb + kBlock - 1) / kBlock;  // raster rows: 256-lane groups
-
   float v = 0.0f, g = 0.0f, vh = 0.0f;
   uint32_t st = 0, fl = FIS_FIRE_OFF, sb = 0;
-  if (act) load_cell(s, idx, v, g, st, fl, sb, vh);
+  if (act) {
+    v = s.v[idx];
+    g = s.g[idx];
+    st = s.st[idx];
+    fl = s.fl[idx];
+    sb = s.sbits[idx];
+    if (s.vhold) vh = s.vhold[idx];  // used only where the flag says HOLD
+  }
+
+  {  // 1. gather, lane = (in-edge, 64-trial word).  The next in-edge and its
+     // act word are fetched while this one is summed; the record loads of
+     // all its active steps are in flight at once.
+    constexpr int kIn = 8;
+    const int wl = tid & (wpt - 1);  // word of this lane within the tile
+    const int w = w0 + wl;
+    constexpr int G = TILE / wpt;    // in-edges per workgroup iteration
+    const bool wlive = !s.word_end || t0 < s.word_end[w];
+    const bool nowrap = rbase + W <= kRec;  // uniform
+    const int rot = (kRec - rbase) % kRec;
+    const int e1 = s.in_offsets[p0 + np];
+    int e = off[0] + tid / wpt;
+    int src = 0;
+    int32_t val = 0;
+    uint64_t a = 0;  // act word of the presynaptic neuron
+    if (e < e1) {
+      src = s.in_cols[e];
+      val = s.in_vals[e];
+      if (wlive) a = s.act[src * parts + part];
+    }
+    while (e < e1) {
+      const int en = e + G;
+      int src_n = 0;
+      int32_t val_n = 0;
+      if (en < e1) {
+        src_n = s.in_cols[en];
+        val_n = s.in_vals[en];
+      }
+      int pq = 0;
+#pragma unroll
+      for (int q = 1; q < P; ++q) pq += e >= off[q] ? 1 : 0;
+      uint64_t m = rot40(a, rot) & wmask;
+      int32_t* ap = acc + pq * tpn + (wl << 6);
+      const uint64_t* rp = s.rec + (int64_t)src * pstride + w;
+      // the horizon's slots rbase .. rbase+W-1 are consecutive unless they
+      // wrap: then step j's word sits at a fixed offset from one base
+      const uint64_t* rq = rp + rbase * words;
+      const uint32_t m32 = (uint32_t)m;
+      uint64_t x[WMAX];
+#pragma unroll
+      for (int q = 0; q < WMAX; ++q)
+        x[q] = (nowrap && ((m32 >> q) & 1u)) ? rq[q * words] : 0ull;
+      const uint64_t a_n =
+          (en < e1 && wlive) ? s.act[src_n * parts + part] : 0ull;
+      if (nowrap) {
+#pragma unroll
+        for (int q = 0; q < WMAX; ++q) add_bits(ap + q * TILE, x[q], val);
+        m = 0;
+      }
+      while (m) {
+        int j[kIn];
+        uint64_t y[kIn];
+#pragma unroll
+        for (int q = 0; q < kIn; ++q) {
+          j[q] = m ? __ffsll((long long)m) - 1 : -1;
+          m &= m - 1ull;
+        }
+#pragma unroll
+        for (int q = 0; q < kIn; ++q) {
+          int sl = rbase + j[q];
+          sl = sl >= kRec ? sl - kRec : sl;
+          y[q] = j[q] >= 0 ? rp[sl * words] : 0ull;
+        }
+#pragma unroll
+        for (int q = 0; q < kIn; ++q) add_bits(ap + j[q] * TILE, y[q], val);
+      }
+      e = en;
+      src = src_n;
+      val = val_n;
+      a = a_n;
+    }
+  }
+  __syncthreads();
+
+  // 2. update
+  if (sb) s.sbits[idx] = 0u;  // the window's kicks are consumed
   Cell c = make_cell(s, fl, vh, st, v, g);
   const Stim stim = make_stim(s);
   const bool transmit = act && !(fl & FIS_OUT_OFF);
