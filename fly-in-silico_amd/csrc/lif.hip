@@ -64,12 +64,12 @@ __device__ __forceinline__ uint32_t bern(uint32_t x, float p) {
 }
 
 // Kick bits of one neuron-trial for steps t0 .. t0+W-1 at a probability
-// that may change per step (p_of(t)).  Counter (t>>2, neuron, trial_id, 0),
-// key = 64-bit seed, word t & 3.
+// that may change per step (p_next() gives the probability of each step in
+// turn).  Counter (t>>2, neuron, trial_id, 0), key = 64-bit seed, word t & 3.
 template <class P>
 __device__ __forceinline__ uint32_t window_bits(int64_t t0, int W,
                                                 uint32_t neuron, uint32_t tid,
-                                                uint64_t seed, P p_of) {
+                                                uint64_t seed, P p_next) {
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
   uint32_t bits = 0;
   U4 x{0, 0, 0, 0};
@@ -77,11 +77,43 @@ __device__ __forceinline__ uint32_t window_bits(int64_t t0, int W,
     const int64_t t = t0 + k;
     if (k == 0 || (t & 3) == 0)
       x = philox(U4{(uint32_t)(t >> 2), neuron, tid, 0u}, k0, k1);
-    const float p = p_of(t);
+    const float p = p_next();
     if (p > 0.0f) bits |= bern(pick(x, (uint32_t)(t & 3)), p) << k;
   }
   return bits;
 }
+
+// The drive table's probability of one group, step by step from trial step
+// t: rate[min(t / seg_steps, n_seg - 1)][group] * prob_scale.  One division
+// per window; the segment then advances by counting.
+struct TableRate {
+  const float* rate;
+  int n_groups, n_seg, seg_steps, group;
+  float ps;
+  int64_t seg;
+  int rem;
+  float p;
+  __device__ TableRate(const float* rate_, int n_groups_, int n_seg_,
+                       int seg_steps_, int group_, float ps_, int64_t t)
+      : rate(rate_), n_groups(n_groups_), n_seg(n_seg_),
+        seg_steps(seg_steps_), group(group_), ps(ps_) {
+    seg = t / seg_steps;
+    rem = (int)(t - seg * seg_steps);
+    p = at(seg);
+  }
+  __device__ float at(int64_t sg) const {
+    sg = sg < 0 ? 0 : (sg < n_seg ? sg : n_seg - 1);
+    return rate[sg * n_groups + group] * ps;
+  }
+  __device__ float operator()() {
+    const float now = p;
+    if (++rem == seg_steps) {
+      rem = 0;
+      p = at(++seg);
+    }
+    return now;
+  }
+};
 
 // trial b's own step count at global step t (0 at its slot's start)
 __device__ __forceinline__ int64_t origin_of(const fis_sim& s, int b) {
@@ -112,8 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_drive_rates(fis_sim s,
     const float p = rates[(int64_t)b * s.n + i] * s.prob_scale;
     if (p > 0.0f)
       bits = window_bits(*s.clock - origin_of(s, b), W, (uint32_t)i,
-                         s.trial_ids[b], s.seeds[b],
-                         [p](int64_t) { return p; });
+                         s.trial_ids[b], s.seeds[b], [p]() { return p; });
   }
   s.sbits[idx] = bits;
 }
@@ -126,14 +157,11 @@ __global__ __launch_bounds__(kBlock) void k_drive_table(
   const fis_drive_entry e = entries[j];
   const int i = (int)e.idx / s.bstride;
   const int b = (int)e.idx - i * s.bstride;
-  const float ps = s.prob_scale;
+  const int64_t t = *s.clock - origin_of(s, b);
   const uint32_t bits = window_bits(
-      *s.clock - origin_of(s, b), W, (uint32_t)i, s.trial_ids[b], s.seeds[b],
-      [&](int64_t t) {
-        int64_t seg = t / seg_steps;
-        if (seg >= n_seg) seg = n_seg - 1;
-        return rate_hz[seg * n_groups + e.group] * ps;
-      });
+      t, W, (uint32_t)i, s.trial_ids[b], s.seeds[b],
+      TableRate(rate_hz, n_groups, n_seg, seg_steps, e.group, s.prob_scale,
+                t));
   // entries of neighbouring neuron-trials may sit in other workgroups: set
   // the bits with an atomic so no two workgroups store into one line
   if (bits) atomicOr(&s.sbits[e.idx], bits);
@@ -273,16 +301,11 @@ __device__ __forceinline__ uint32_t fused_kicks(const fis_sim& s, int idx,
                                                 int64_t t0, int W) {
   const int gr = s.drive_group[idx];
   if (gr < 0) return 0u;
-  const float ps = s.prob_scale;
-  const int seg_steps = s.drive_seg_steps, n_seg = s.drive_n_seg;
-  const int n_groups = s.drive_n_groups;
-  const float* rate = s.drive_rate;
-  return window_bits(t0 - origin_of(s, trial), W, (uint32_t)neuron,
-                     s.trial_ids[trial], s.seeds[trial], [&](int64_t t) {
-                       int64_t seg = t / seg_steps;
-                       if (seg >= n_seg) seg = n_seg - 1;
-                       return rate[seg * n_groups + gr] * ps;
-                     });
+  const int64_t t = t0 - origin_of(s, trial);
+  return window_bits(t, W, (uint32_t)neuron, s.trial_ids[trial],
+                     s.seeds[trial],
+                     TableRate(s.drive_rate, s.drive_n_groups, s.drive_n_seg,
+                               s.drive_seg_steps, gr, s.prob_scale, t));
 }
 
 __device__ __forceinline__ Cell make_cell(const fis_sim& s, uint32_t fl,
