@@ -63,22 +63,33 @@ __device__ __forceinline__ uint32_t bern(uint32_t x, float p) {
   return u < p ? 1u : 0u;
 }
 
-// Kick bits of one neuron-trial for steps t0 .. t0+W-1 at a probability
-// that may change per step (p_next() gives the probability of each step in
-// turn).  Counter (t>>2, neuron, trial_id, 0), key = 64-bit seed, word t & 3.
-template <class P>
+// Kick bits of one neuron-trial for steps t0 .. t0+W-1 (W <= WMAX) at a
+// probability that may change per step (p_next() gives the probability of
+// each step in turn).  Counter (t>>2, neuron, trial_id, 0), key = 64-bit
+// seed, word t & 3.  The window's Philox blocks are independent of each
+// other and computed side by side (one block's latency, not six).
+template <int WMAX, class P>
 __device__ __forceinline__ uint32_t window_bits(int64_t t0, int W,
                                                 uint32_t neuron, uint32_t tid,
                                                 uint64_t seed, P p_next) {
+  constexpr int NC = (WMAX + 3) / 4 + 1;  // blocks a window can touch
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  const int64_t c0 = t0 >> 2;
+  const int off = (int)(t0 & 3);        // word of step t0 in block c0
+  const int nc = (off + W + 3) >> 2;    // blocks this window touches
+  U4 x[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+    x[c] = c < nc ? philox(U4{(uint32_t)(c0 + c), neuron, tid, 0u}, k0, k1)
+                  : U4{0u, 0u, 0u, 0u};
   uint32_t bits = 0;
-  U4 x{0, 0, 0, 0};
-  for (int k = 0; k < W; ++k) {
-    const int64_t t = t0 + k;
-    if (k == 0 || (t & 3) == 0)
-      x = philox(U4{(uint32_t)(t >> 2), neuron, tid, 0u}, k0, k1);
-    const float p = p_next();
-    if (p > 0.0f) bits |= bern(pick(x, (uint32_t)(t & 3)), p) << k;
+#pragma unroll
+  for (int i = 0; i < 4 * NC; ++i) {  // word i of the blocks = step i - off
+    const int k = i - off;
+    if (k >= 0 && k < W) {
+      const float p = p_next();
+      if (p > 0.0f) bits |= bern(pick(x[i >> 2], (uint32_t)(i & 3)), p) << k;
+    }
   }
   return bits;
 }
@@ -143,8 +154,9 @@ __global__ __launch_bounds__(kBlock) void k_drive_rates(fis_sim s,
   if (b < s.batch) {
     const float p = rates[(int64_t)b * s.n + i] * s.prob_scale;
     if (p > 0.0f)
-      bits = window_bits(*s.clock - origin_of(s, b), W, (uint32_t)i,
-                         s.trial_ids[b], s.seeds[b], [p]() { return p; });
+      bits = window_bits<FIS_MAX_WINDOW>(*s.clock - origin_of(s, b), W,
+                                         (uint32_t)i, s.trial_ids[b],
+                                         s.seeds[b], [p]() { return p; });
   }
   s.sbits[idx] = bits;
 }
@@ -158,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void k_drive_table(
   const int i = (int)e.idx / s.bstride;
   const int b = (int)e.idx - i * s.bstride;
   const int64_t t = *s.clock - origin_of(s, b);
-  const uint32_t bits = window_bits(
+  const uint32_t bits = window_bits<FIS_MAX_WINDOW>(
       t, W, (uint32_t)i, s.trial_ids[b], s.seeds[b],
       TableRate(rate_hz, n_groups, n_seg, seg_steps, e.group, s.prob_scale,
                 t));
@@ -296,16 +308,38 @@ __device__ __forceinline__ void load_cell(const fis_sim& s, int idx, float& v,
 
 // The window's kicks of a table-driven neuron-trial (fused drive): the
 // bits k_drive_table would set for it.
-__device__ __forceinline__ uint32_t fused_kicks(const fis_sim& s, int idx,
-                                                int neuron, int trial,
+// What the fused drive of one neuron-trial reads: its group (< 0: not
+// driven) and its trial's Philox identity and origin.  Read at the start
+// of the update, all at once, so that they arrive with the state.
+struct DriveId {
+  int group;
+  uint32_t trial_id;
+  uint64_t seed;
+  int64_t origin;
+};
+
+__device__ __forceinline__ DriveId drive_id(const fis_sim& s, int idx,
+                                            int trial) {
+  DriveId d{-1, 0u, 0ull, 0};
+  if (s.drive_group) {
+    d.group = s.drive_group[idx];
+    d.trial_id = s.trial_ids[trial];
+    d.seed = s.seeds[trial];
+    d.origin = origin_of(s, trial);
+  }
+  return d;
+}
+
+template <int WMAX>
+__device__ __forceinline__ uint32_t fused_kicks(const fis_sim& s,
+                                                const DriveId& d, int neuron,
                                                 int64_t t0, int W) {
-  const int gr = s.drive_group[idx];
-  if (gr < 0) return 0u;
-  const int64_t t = t0 - origin_of(s, trial);
-  return window_bits(t, W, (uint32_t)neuron, s.trial_ids[trial],
-                     s.seeds[trial],
-                     TableRate(s.drive_rate, s.drive_n_groups, s.drive_n_seg,
-                               s.drive_seg_steps, gr, s.prob_scale, t));
+  if (d.group < 0) return 0u;
+  const int64_t t = t0 - d.origin;
+  return window_bits<WMAX>(t, W, (uint32_t)neuron, d.trial_id, d.seed,
+                           TableRate(s.drive_rate, s.drive_n_groups,
+                                     s.drive_n_seg, s.drive_seg_steps,
+                                     d.group, s.prob_scale, t));
 }
 
 __device__ __forceinline__ Cell make_cell(const fis_sim& s, uint32_t fl,
@@ -399,19 +433,11 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
   const int64_t t0 = *s.clock;
   const int pre = idx / bs;       // neuron of this lane
   const int tb = idx - pre * bs;  // trial of this lane
-
-  float v = 0.0f, g = 0.0f, vh = 0.0f;
-  uint32_t st = 0, fl = FIS_FIRE_OFF, sb = 0;
-  if (act) load_cell(s, idx, v, g, st, fl, sb, vh);
-  if (act && s.drive_group) sb |= fused_kicks(s, idx, pre, tb, t0, W);
-
-  Cell c = make_cell(s, fl, vh, st, v, g);
-  const Stim stim = make_stim(s);
-  const bool transmit = act && !(fl & FIS_OUT_OFF);
-  const int64_t t_stop = act ? end_of(s, tb) : 0;
+  const DriveId drive = act ? drive_id(s, idx, tb) : DriveId{-1, 0u, 0ull, 0};
 
   // the window's ring slots, all read (and the non-zero ones cleared)
-  // before the recurrence starts: nothing in it waits for memory
+  // before the recurrence starts: nothing in it waits for memory, and the
+  // loads are in flight while the drive's Philox blocks are computed
   int32_t* rp = s.ring + idx;
   const int slot0 = (int)((t0 + 1) % FIS_RING_LEN);
   int32_t in[WMAX];
@@ -422,7 +448,18 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
       in[k] = (k < W && act) ? rp[(int64_t)sl * nb] : 0;
       sl = (sl + 1 == FIS_RING_LEN) ? 0 : sl + 1;
     }
-    sl = slot0;
+  }
+  float v = 0.0f, g = 0.0f, vh = 0.0f;
+  uint32_t st = 0, fl = FIS_FIRE_OFF, sb = 0;
+  if (act) load_cell(s, idx, v, g, st, fl, sb, vh);
+  sb |= fused_kicks<WMAX>(s, drive, pre, t0, W);
+
+  Cell c = make_cell(s, fl, vh, st, v, g);
+  const Stim stim = make_stim(s);
+  const bool transmit = act && !(fl & FIS_OUT_OFF);
+  const int64_t t_stop = act ? end_of(s, tb) : 0;
+  {
+    int sl = slot0;
 #pragma unroll
     for (int k = 0; k < WMAX; ++k) {
       if (in[k] != 0) rp[(int64_t)sl * nb] = 0;
