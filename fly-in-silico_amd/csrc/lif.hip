@@ -735,71 +735,98 @@ void k_pull(fis_sim s, int W,
     if (s.vhold) vh = s.vhold[idx];  // used only where the flag says HOLD
   }
 
-  {  // 1. gather, lane = (in-edge, 64-trial word).  The next in-edge and its
-     // act word are fetched while this one is summed; the record loads of
-     // all its active steps are in flight at once.
-    constexpr int kIn = 8;
-    const int wl = tid & (wpt - 1);  // word of this lane within the tile
-    const int w = w0 + wl;
-    constexpr int G = TILE / wpt;    // in-edges per workgroup iteration
-    const bool wlive = !s.word_end || t0 < s.word_end[w];
-    const bool nowrap = rbase + W <= kRec;  // uniform
-    const int rot = (kRec - rbase) % kRec;
+  {  // 1. gather.  A wave takes 64 in-edges at a time, one per lane, and
+     // reads their presynaptic act words (the next 64 are fetched
+     // meanwhile).  The active (in-edge, step) pairs of those 64 -- a step
+     // of the horizon at which the presynaptic neuron spiked in some trial
+     // of the tile -- are then dealt out 64 at a time, one per lane: the
+     // lane reads that step's record words of the tile's trials and adds
+     // the synapse count at every set trial bit.  A lane thus carries a few
+     // trial bits of one spike of one neuron, and the walk over them is as
+     // long as the densest such pair of the round (a handful of bits), not
+     // as long as the densest of 20 steps' words of the in-edge.
+    const int wv = tid >> 6;                 // wave of the tile
+    constexpr int NW = TILE / kWave;         // waves per tile
+    uint32_t live_words = 0;                 // tile words still simulated
+#pragma unroll
+    for (int j = 0; j < wpt; ++j)
+      if (!s.word_end || t0 < s.word_end[w0 + j]) live_words |= 1u << j;
     const int e1 = s.in_offsets[p0 + np];
-    int e = off[0] + tid / wpt;
+    int e = off[0] + wv * kWave + lane;      // this lane's in-edge
     int src = 0;
     int32_t val = 0;
     uint64_t a = 0;  // act word of the presynaptic neuron
     if (e < e1) {
       src = s.in_cols[e];
       val = s.in_vals[e];
-      if (wlive) a = s.act[src * parts + part];
+      a = s.act[src * parts + part];
     }
-    while (e < e1) {
-      const int en = e + G;
+    const int rot = (kRec - rbase) % kRec;
+    // uniform over the wave: its first in-edge is below e1
+    while (e - lane < e1) {
+      const int en = e + NW * kWave;
       int src_n = 0;
       int32_t val_n = 0;
       if (en < e1) {
         src_n = s.in_cols[en];
         val_n = s.in_vals[en];
       }
-      int pq = 0;
+      int pq = 0;  // position of the in-edge's postsynaptic neuron
 #pragma unroll
       for (int q = 1; q < P; ++q) pq += e >= off[q] ? 1 : 0;
-      uint64_t m = rot40(a, rot) & wmask;
-      int32_t* ap = acc + pq * tpn + (wl << 6);
-      const uint64_t* rp = s.rec + (int64_t)src * pstride + w;
-      // the horizon's slots rbase .. rbase+W-1 are consecutive unless they
-      // wrap: then step j's word sits at a fixed offset from one base
-      const uint64_t* rq = rp + rbase * words;
-      const uint32_t m32 = (uint32_t)m;
-      uint64_t x[WMAX];
+      const uint32_t m = (e < e1 && live_words)
+                             ? (uint32_t)(rot40(a, rot) & wmask) : 0u;
+      const uint64_t a_n = en < e1 ? s.act[src_n * parts + part] : 0ull;
+      // exclusive prefix of the lanes' active steps, and their total
+      const int pc = __popc(m);
+      int incl = pc;
 #pragma unroll
-      for (int q = 0; q < WMAX; ++q)
-        x[q] = (nowrap && ((m32 >> q) & 1u)) ? rq[q * words] : 0ull;
-      const uint64_t a_n =
-          (en < e1 && wlive) ? s.act[src_n * parts + part] : 0ull;
-      if (nowrap) {
-#pragma unroll
-        for (int q = 0; q < WMAX; ++q) add_bits(ap + q * TILE, x[q], val);
-        m = 0;
+      for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        incl += lane >= o ? y : 0;
       }
-      while (m) {
-        int j[kIn];
-        uint64_t y[kIn];
+      const int total = __shfl(incl, kWave - 1);
+      const int excl = incl - pc;
+      for (int base = 0; base < total; base += kWave) {  // uniform
+        const int pi = base + lane;  // the pair this lane takes
+        // its owner: the last lane whose first pair is <= pi
+        int o = 0, eo = 0;
 #pragma unroll
-        for (int q = 0; q < kIn; ++q) {
-          j[q] = m ? __ffsll((long long)m) - 1 : -1;
-          m &= m - 1ull;
+        for (int st = kWave / 2; st > 0; st >>= 1) {
+          const int ex = __shfl(excl, o + st);
+          if (ex <= pi) {
+            o += st;
+            eo = ex;
+          }
         }
+        const uint32_t mo = (uint32_t)__shfl((int)m, o);
+        const int so = __shfl(src, o);
+        const int32_t vo = __shfl(val, o);
+        const int po = __shfl(pq, o);
+        if (pi < total) {
+          // the step: the (pi - eo)-th set bit of the owner's mask
+          int k = pi - eo, j = 0;
+          uint32_t x = mo;
 #pragma unroll
-        for (int q = 0; q < kIn; ++q) {
-          int sl = rbase + j[q];
+          for (int wd = 16; wd > 0; wd >>= 1) {
+            const int c = __popc(x & ((1u << wd) - 1u));
+            if (k >= c) {
+              k -= c;
+              x >>= wd;
+              j += wd;
+            }
+          }
+          int sl = rbase + j;
           sl = sl >= kRec ? sl - kRec : sl;
-          y[q] = j[q] >= 0 ? rp[sl * words] : 0ull;
-        }
+          const uint64_t* rw = s.rec + (int64_t)so * pstride + sl * words + w0;
+          uint64_t xw[wpt];
 #pragma unroll
-        for (int q = 0; q < kIn; ++q) add_bits(ap + j[q] * TILE, y[q], val);
+          for (int q = 0; q < wpt; ++q)
+            xw[q] = ((live_words >> q) & 1u) ? rw[q] : 0ull;
+          int32_t* at = acc + j * TILE + po * tpn;
+#pragma unroll
+          for (int q = 0; q < wpt; ++q) add_bits(at + q * kWave, xw[q], vo);
+        }
       }
       e = en;
       src = src_n;
