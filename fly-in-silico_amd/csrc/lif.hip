@@ -651,6 +651,50 @@ __device__ __forceinline__ void add_bits(int32_t* acc, uint64_t x,
   }
 }
 
+// One active (in-edge, step) pair of a wave's 64 in-edges, dealt to the
+// lane that takes pair number pi: lane L of the 64 owns pairs
+// excl(L) .. excl(L) + popc(m(L)) - 1, one per set step bit of m(L).  All
+// 64 lanes must take part (the owner is found by a binary search over the
+// lanes' excl, read by shuffles).
+struct Pair {
+  int src;      // presynaptic neuron
+  int32_t val;  // synapse count
+  int pos;      // position of the postsynaptic neuron in the tile
+  int step;     // step of the horizon (0 .. W-1)
+};
+
+__device__ __forceinline__ Pair deal_pair(int pi, int excl, uint32_t m,
+                                          int src, int32_t val, int pos) {
+  int o = 0, eo = 0;
+#pragma unroll
+  for (int st = kWave / 2; st > 0; st >>= 1) {
+    const int ex = __shfl(excl, o + st);
+    if (ex <= pi) {
+      o += st;
+      eo = ex;
+    }
+  }
+  Pair p;
+  const uint32_t mo = (uint32_t)__shfl((int)m, o);
+  p.src = __shfl(src, o);
+  p.val = __shfl(val, o);
+  p.pos = __shfl(pos, o);
+  // the step: the (pi - eo)-th set bit of the owner's mask
+  int k = pi - eo, j = 0;
+  uint32_t x = mo;
+#pragma unroll
+  for (int wd = 16; wd > 0; wd >>= 1) {
+    const int c = __popc(x & ((1u << wd) - 1u));
+    if (k >= c) {
+      k -= c;
+      x >>= wd;
+      j += wd;
+    }
+  }
+  p.step = j;
+  return p;
+}
+
 // Neuron-trials per workgroup for a trial stride.  256 lanes (4 waves,
 // 20 KB of accumulators) measured fastest at 256 trials: smaller
 // workgroups wait less at the barrier between gather and update for their
@@ -787,45 +831,30 @@ void k_pull(fis_sim s, int W,
       }
       const int total = __shfl(incl, kWave - 1);
       const int excl = incl - pc;
-      for (int base = 0; base < total; base += kWave) {  // uniform
-        const int pi = base + lane;  // the pair this lane takes
-        // its owner: the last lane whose first pair is <= pi
-        int o = 0, eo = 0;
+      // two rounds at a time: their owner searches and record loads overlap
+      for (int base = 0; base < total; base += 2 * kWave) {  // uniform
+        Pair pr[2];
 #pragma unroll
-        for (int st = kWave / 2; st > 0; st >>= 1) {
-          const int ex = __shfl(excl, o + st);
-          if (ex <= pi) {
-            o += st;
-            eo = ex;
-          }
-        }
-        const uint32_t mo = (uint32_t)__shfl((int)m, o);
-        const int so = __shfl(src, o);
-        const int32_t vo = __shfl(val, o);
-        const int po = __shfl(pq, o);
-        if (pi < total) {
-          // the step: the (pi - eo)-th set bit of the owner's mask
-          int k = pi - eo, j = 0;
-          uint32_t x = mo;
+        for (int r = 0; r < 2; ++r)
+          pr[r] = deal_pair(base + r * kWave + lane, excl, m, src, val, pq);
+        uint64_t xw[2][wpt];
 #pragma unroll
-          for (int wd = 16; wd > 0; wd >>= 1) {
-            const int c = __popc(x & ((1u << wd) - 1u));
-            if (k >= c) {
-              k -= c;
-              x >>= wd;
-              j += wd;
-            }
-          }
-          int sl = rbase + j;
+        for (int r = 0; r < 2; ++r) {
+          const bool ok = base + r * kWave + lane < total;
+          int sl = rbase + pr[r].step;
           sl = sl >= kRec ? sl - kRec : sl;
-          const uint64_t* rw = s.rec + (int64_t)so * pstride + sl * words + w0;
-          uint64_t xw[wpt];
+          const uint64_t* rw =
+              s.rec + (int64_t)pr[r].src * pstride + sl * words + w0;
 #pragma unroll
           for (int q = 0; q < wpt; ++q)
-            xw[q] = ((live_words >> q) & 1u) ? rw[q] : 0ull;
-          int32_t* at = acc + j * TILE + po * tpn;
+            xw[r][q] = (ok && ((live_words >> q) & 1u)) ? rw[q] : 0ull;
+        }
 #pragma unroll
-          for (int q = 0; q < wpt; ++q) add_bits(at + q * kWave, xw[q], vo);
+        for (int r = 0; r < 2; ++r) {
+          int32_t* at = acc + pr[r].step * TILE + pr[r].pos * tpn;
+#pragma unroll
+          for (int q = 0; q < wpt; ++q)
+            add_bits(at + q * kWave, xw[r][q], pr[r].val);
         }
       }
       e = en;
