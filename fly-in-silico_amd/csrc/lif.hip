@@ -779,8 +779,8 @@ void k_pull(fis_sim s, int W,
     if (s.vhold) vh = s.vhold[idx];  // used only where the flag says HOLD
   }
 
-  {  // 1. gather.  A wave takes 64 in-edges at a time, one per lane, and
-     // reads their presynaptic act words (the next 64 are fetched
+  {  // 1. gather.  The tile's in-edges are read 64 at a time, one per lane,
+     // with their presynaptic act words (the next 64 are fetched
      // meanwhile).  The active (in-edge, step) pairs of those 64 -- a step
      // of the horizon at which the presynaptic neuron spiked in some trial
      // of the tile -- are then dealt out 64 at a time, one per lane: the
@@ -796,7 +796,13 @@ void k_pull(fis_sim s, int W,
     for (int j = 0; j < wpt; ++j)
       if (!s.word_end || t0 < s.word_end[w0 + j]) live_words |= 1u << j;
     const int e1 = s.in_offsets[p0 + np];
-    int e = off[0] + wv * kWave + lane;      // this lane's in-edge
+    // one neuron per tile: every wave reads the same 64 in-edges and the
+    // waves share their rounds of pairs (balanced however few the edges);
+    // several neurons per tile: each wave takes its own 64 in-edges
+    constexpr bool kShare = P == 1;
+    constexpr int kEdgeStep = kShare ? kWave : NW * kWave;
+    constexpr int kRoundGap = kShare ? NW * kWave : kWave;
+    int e = off[0] + (kShare ? 0 : wv * kWave) + lane;  // this lane's in-edge
     int src = 0;
     int32_t val = 0;
     uint64_t a = 0;  // act word of the presynaptic neuron
@@ -808,7 +814,7 @@ void k_pull(fis_sim s, int W,
     const int rot = (kRec - rbase) % kRec;
     // uniform over the wave: its first in-edge is below e1
     while (e - lane < e1) {
-      const int en = e + NW * kWave;
+      const int en = e + kEdgeStep;
       int src_n = 0;
       int32_t val_n = 0;
       if (en < e1) {
@@ -831,16 +837,18 @@ void k_pull(fis_sim s, int W,
       }
       const int total = __shfl(incl, kWave - 1);
       const int excl = incl - pc;
-      // two rounds at a time: their owner searches and record loads overlap
-      for (int base = 0; base < total; base += 2 * kWave) {  // uniform
+      // rounds of 64 pairs, two at a time (their owner searches and record
+      // loads overlap); shared batches hand the rounds to the waves in turn
+      for (int base = kShare ? wv * kWave : 0; base < total;
+           base += 2 * kRoundGap) {
         Pair pr[2];
 #pragma unroll
         for (int r = 0; r < 2; ++r)
-          pr[r] = deal_pair(base + r * kWave + lane, excl, m, src, val, pq);
+          pr[r] = deal_pair(base + r * kRoundGap + lane, excl, m, src, val, pq);
         uint64_t xw[2][wpt];
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
-          const bool ok = base + r * kWave + lane < total;
+          const bool ok = base + r * kRoundGap + lane < total;
           int sl = rbase + pr[r].step;
           sl = sl >= kRec ? sl - kRec : sl;
           const uint64_t* rw =
