@@ -138,6 +138,7 @@ def test_small10k_config1(golden):
                                               (64, "push", False),
                                               (130, "push", False),
                                               (64, "pull", False),
+                                              (100, "pull", False),
                                               (130, "pull", False),
                                               (256, "pull", False),
                                               (300, "pull", False),
@@ -178,7 +179,7 @@ def test_philox_drive_batched(batch, road, fused):
     compare_state(sim, ora, "end")
 
 
-@pytest.mark.parametrize("batch", [64, 200, 600])
+@pytest.mark.parametrize("batch", [64, 100, 200, 600])
 def test_pull_road_uneven_windows_and_out_masks(batch):
     """Pull road with a partial last tile (509 neurons), windows of 1..20
     steps (records of one delay horizon span several windows) and m_out
