@@ -419,6 +419,33 @@ __device__ __forceinline__ void store_cell(const fis_sim& s, const Cell& c,
   }
 }
 
+// W steps of lif_step with the inputs in registers; returns the steps at
+// which the neuron-trial fired (bit k = step k).  A full window runs as one
+// unrolled chain with no per-step test of W.
+template <int WMAX>
+__device__ __forceinline__ uint32_t run_window(const fis_sim& s, Cell& c,
+                                               const int32_t (&in)[WMAX],
+                                               uint32_t kicks, Stim stim,
+                                               int W) {
+  // the two stimulus values stay materialised in registers (the compiler
+  // would otherwise redo w_scale * (...) at every step)
+  asm volatile("" : "+v"(stim.on), "+v"(stim.off));
+  uint32_t fired = 0;
+  if (W == WMAX) {
+#pragma unroll
+    for (int k = 0; k < WMAX; ++k)
+      fired |= lif_step(s, c, in[k], (kicks >> k) & 1u, stim) ? (1u << k)
+                                                                : 0u;
+  } else {
+#pragma unroll
+    for (int k = 0; k < WMAX; ++k)
+      if (k < W)
+        fired |= lif_step(s, c, in[k], (kicks >> k) & 1u, stim) ? (1u << k)
+                                                                  : 0u;
+  }
+  return fired;
+}
+
 // W steps per neuron-trial.  WMAX is the unrolled bound (1 or 20).
 // Index arithmetic is 32-bit: n * bstride < 2^31 is checked on the host.
 template <int WMAX>
@@ -467,16 +494,12 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
     }
   }
 
-  uint32_t fired_bits = 0;  // bit k: this lane fired at step k
-  uint32_t nspk = 0;
-#pragma unroll
-  for (int k = 0; k < WMAX; ++k) {
-    if (k < W) {
-      const bool fired = lif_step(s, c, in[k], (sb >> k) & 1u, stim) && act;
-      nspk += (fired && t0 + k < t_stop) ? 1u : 0u;
-      fired_bits |= fired ? (1u << k) : 0u;
-    }
-  }
+  // bit k: this lane fired at step k
+  const uint32_t fired_bits =
+      act ? run_window<WMAX>(s, c, in, sb, stim, W) : 0u;
+  const int64_t left = t_stop - t0;  // steps whose spikes still count
+  const int kstop = left <= 0 ? 0 : (left >= W ? W : (int)left);
+  const uint32_t nspk = __popc(fired_bits & ((1u << kstop) - 1u));
 
   // spikes out, step by step: raster rows, and events (below 64 trials
   // only steps with spikes append any, so a wave that stayed silent for the
@@ -903,16 +926,10 @@ void k_pull(fis_sim s, int W,
     }
   }
 
-  uint32_t fired_bits = 0;  // bit k: this lane fired at step k
-  uint32_t nspk = 0;
-#pragma unroll
-  for (int k = 0; k < WMAX; ++k) {
-    if (k < W) {
-      const bool fired = lif_step(s, c, in[k], (sb >> k) & 1u, stim) && act;
-      nspk += (fired && k < kstop) ? 1u : 0u;
-      fired_bits |= fired ? (1u << k) : 0u;
-    }
-  }
+  // bit k: this lane fired at step k
+  const uint32_t fired_bits =
+      act ? run_window<WMAX>(s, c, in, sb, stim, W) : 0u;
+  const uint32_t nspk = __popc(fired_bits & ((1u << kstop) - 1u));
   if (act) store_cell(s, c, idx, pre, nspk);
 
   // The wave's record words of the window (and raster rows): lane k < W
