@@ -850,16 +850,20 @@ void k_pull(fis_sim s, int W,
       const uint32_t m = (e < e1 && live_words)
                              ? (uint32_t)(rot40(a, rot) & wmask) : 0u;
       const uint64_t a_n = en < e1 ? s.act[src_n * parts + part] : 0ull;
-      // exclusive prefix of the lanes' active steps, and their total
+      // exclusive prefix of the lanes' active steps, and their total, bit
+      // by bit of the counts (<= 20): a ballot per bit, counted below each
+      // lane (no round trip through LDS)
       const int pc = __popc(m);
-      int incl = pc;
+      int excl = 0, total = 0;
 #pragma unroll
-      for (int o = 1; o < kWave; o <<= 1) {
-        const int y = __shfl_up(incl, o);
-        incl += lane >= o ? y : 0;
+      for (int b = 0; b < 5; ++b) {
+        const uint64_t mb = __ballot((pc >> b) & 1);
+        excl += (int)__builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(mb >> 32),
+                    __builtin_amdgcn_mbcnt_lo((uint32_t)mb, 0u))
+                << b;
+        total += __popcll(mb) << b;
       }
-      const int total = __shfl(incl, kWave - 1);
-      const int excl = incl - pc;
       // rounds of 64 pairs, two at a time (their owner searches and record
       // loads overlap); shared batches hand the rounds to the waves in turn
       for (int base = kShare ? wv * kWave : 0; base < total;
