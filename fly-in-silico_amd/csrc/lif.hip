@@ -718,6 +718,12 @@ __device__ __forceinline__ Pair deal_pair(int pi, int excl, uint32_t m,
   return p;
 }
 
+// Waves of a one-neuron tile that read the same in-edges and share their
+// rounds of pairs.  Measured at 256 trials, window ms: 1 (each wave its own
+// in-edges) 1.136; 4 (all waves) 0.98; 2 0.918 -- pairs of waves balance the
+// typical 100-odd in-edges and still give each wave two rounds to overlap.
+constexpr int kPullShare = 2;
+
 // Neuron-trials per workgroup for a trial stride.  256 lanes (4 waves,
 // 20 KB of accumulators) measured fastest at 256 trials: smaller
 // workgroups wait less at the barrier between gather and update for their
@@ -819,13 +825,15 @@ void k_pull(fis_sim s, int W,
     for (int j = 0; j < wpt; ++j)
       if (!s.word_end || t0 < s.word_end[w0 + j]) live_words |= 1u << j;
     const int e1 = s.in_offsets[p0 + np];
-    // one neuron per tile: every wave reads the same 64 in-edges and the
-    // waves share their rounds of pairs (balanced however few the edges);
-    // several neurons per tile: each wave takes its own 64 in-edges
-    constexpr bool kShare = P == 1;
-    constexpr int kEdgeStep = kShare ? kWave : NW * kWave;
-    constexpr int kRoundGap = kShare ? NW * kWave : kWave;
-    int e = off[0] + (kShare ? 0 : wv * kWave) + lane;  // this lane's in-edge
+    // one neuron per tile: the waves form groups of SG that read the same
+    // 64 in-edges and share their rounds of pairs (balanced however few the
+    // edges); several neurons per tile: each wave takes its own 64 in-edges
+    constexpr int SG = P == 1 ? kPullShare : 1;  // waves per group
+    constexpr int NG = NW / SG;                  // groups per tile
+    const int grp = wv / SG, mem = wv % SG;
+    constexpr int kEdgeStep = NG * kWave;
+    constexpr int kRoundGap = SG * kWave;
+    int e = off[0] + grp * kWave + lane;  // this lane's in-edge
     int src = 0;
     int32_t val = 0;
     uint64_t a = 0;  // act word of the presynaptic neuron
@@ -866,8 +874,7 @@ void k_pull(fis_sim s, int W,
       }
       // rounds of 64 pairs, two at a time (their owner searches and record
       // loads overlap); shared batches hand the rounds to the waves in turn
-      for (int base = kShare ? wv * kWave : 0; base < total;
-           base += 2 * kRoundGap) {
+      for (int base = mem * kWave; base < total; base += 2 * kRoundGap) {
         Pair pr[2];
 #pragma unroll
         for (int r = 0; r < 2; ++r)
