@@ -9,12 +9,13 @@
 //    with all their trials; a spike ballot is then directly the trial mask
 //    of an event, and the scatter reads a CSR row once for every trial in
 //    that mask;
-//  * the delay ring is int32 synapse counts, 20 slots: the update of step t
-//    reads and clears slot (t+1) mod 20, which holds the input computed from
-//    the spikes of step t-20 (reference: rec written at step u, read at
-//    u+19); the scatter of step t's spikes adds into the same slot.  Since
-//    an input needs 20 steps to arrive, W <= 20 steps can run back to back
-//    with v, g and the refractory counter in registers before one scatter;
+//  * the delay ring is int32 synapse counts, 40 slots (two delays): the
+//    update of step t reads and clears slot (t+1) mod 40, which holds the
+//    input computed from the spikes of step t-20 (reference: rec written at
+//    step u, read at u+19); the spikes of step t are added into slot
+//    (t+21) mod 40.  Since an input needs 20 steps to arrive, W <= 20 steps
+//    can run back to back with v, g and the refractory counter in registers,
+//    and the slots a window reads are never the slots its spikes go to;
 //  * integer atomics make the sum exact and order-free; wScale is applied
 //    once per read, after the sum (sim.py:906-918);
 //  * every float32 operation of the reference is kept as one rounding in
@@ -30,6 +31,7 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
+constexpr int kDelay = 20;  // steps from a spike to the input it makes
 
 // ------------------------------------------------------------- Philox
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
@@ -508,7 +510,9 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
   int nev = 0;  // wave-uniform
   if (s.raster || bs >= kWave || any_lane(fired_bits != 0u)) {
     int rrow = s.raster ? (int)(t0 % s.raster_len) : 0;
-    int slot = slot0;
+    // the spikes of step t0 + k are summed into slot (t0 + k + 21) mod 40,
+    // which step t0 + k + 20 reads
+    int slot = (int)((t0 + kDelay + 1) % FIS_RING_LEN);
 #pragma unroll 1
     for (int k = 0; k < W; ++k) {
       const bool fired = (fired_bits >> k) & 1u;
@@ -1044,7 +1048,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_to_ring(fis_sim s,
   const int slot = (int)(u % kRec);
   const int words = s.bstride >> 6;
   const int64_t nb = (int64_t)s.n * s.bstride;
-  int32_t* base = ring + (int64_t)((u + 1) % FIS_RING_LEN) * nb;
+  int32_t* base = ring + (int64_t)((u + kDelay + 1) % FIS_RING_LEN) * nb;
   const int parts = pull_parts(s.bstride);
   const int wpp = words / parts;  // words per part
   for (int w = 0; w < words; ++w) {

@@ -20,10 +20,21 @@ CSRC = PKG.parent / "csrc"
 SOURCES = [CSRC / "lif.hip"]
 HEADER = ROOT / "include" / "fis.h"
 LIB_PATH = PKG / "libfis.so"
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 MAX_WINDOW = 20
-RING_LEN = 20
+RING_LEN = 40  # int32 ring slots (two delays)
+DELAY = 20     # steps from a spike to the input it makes
+
+
+def write_slot(u):
+    """Ring slot that the spikes of step ``u`` are summed into."""
+    return (u + DELAY + 1) % RING_LEN
+
+
+def read_slot(t):
+    """Ring slot that step ``t`` reads (the spikes of step t - DELAY)."""
+    return (t + 1) % RING_LEN
 FIRE_OFF, OUT_OFF, IN_OFF, HOLD, NOREFRAC = 0x01, 0x02, 0x04, 0x08, 0x10
 BLOCK_WAVES, LINE_WORDS, LINE_INTS, EVENT_ALIGN = 4, 16, 32, 8
 REC_SLOTS, PULL_TILE, PULL_MAX = 40, 256, 1024

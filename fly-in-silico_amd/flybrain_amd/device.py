@@ -8,8 +8,8 @@ device memory and streams only; all arithmetic of the step is in the HIP
 kernels.
 
 Memory per neuron-trial: v, g (4 B each), st, fl (1 B each), sbits (4 B),
-ring 20 x 4 B -- about 94 B; at 138,639 neurons x 256 trials that is
-3.3 GB plus events, far inside one MI355X's 288 GB.
+ring 40 x 4 B -- about 174 B; at 138,639 neurons x 256 trials that is
+6.2 GB plus events, far inside one MI355X's 288 GB.
 """
 
 from __future__ import annotations
@@ -327,7 +327,7 @@ class DeviceSim:
                     "fis_pull_resend")
         else:
             L.check(self.lib.fis_resend(self._sp, _ptr(idx), int(idx.numel()),
-                                        self.t % L.RING_LEN, _stream()),
+                                        L.write_slot(self.t - 1), _stream()),
                     "fis_resend")
         self._resend_idx = idx  # kept alive until the launch has run
 
@@ -554,7 +554,7 @@ class DeviceSim:
 
     def ring_image(self):
         """The input in flight as the push road's int32 ring
-        [20 * n * bstride] (spikes of step u in slot (u+1) mod 20): the
+        [40 * n * bstride] (spikes of step u in slot (u+21) mod 40): the
         ring itself on the push road; on the pull road the carried-over
         ring plus the spike records of the last 20 steps scattered into a
         copy (fis_pull_to_ring)."""
@@ -569,7 +569,7 @@ class DeviceSim:
         """Integer synaptic input that step ``step_input`` reads, i.e.
         the sum computed from the spikes of step ``step_input - 20``;
         [batch, n]."""
-        slot = (step_input + 1) % L.RING_LEN
+        slot = L.read_slot(step_input)
         return self.view(self.ring_image()[slot * self.nb:(slot + 1) * self.nb])
 
     def read_raster(self, t_from, t_to):
@@ -591,7 +591,7 @@ class DeviceSim:
 
     def rec_slot(self, u):
         """The reference's rec(u) (sum of spikes of step u-1), int [B, n]."""
-        slot = u % L.RING_LEN
+        slot = L.write_slot(u - 1)
         return self.view(self.ring_image()[slot * self.nb:(slot + 1) * self.nb])
 
 

@@ -123,9 +123,9 @@ class Scheduler:
 
 def bytes_per_trial(n, road):
     """HBM held per trial slot: v, g, vhold (12 B), st, fl (2 B), int32
-    ring (80 B), kicks and counts (8 B), events (6 B) and, on the pull
+    ring (160 B), kicks and counts (8 B), events (6 B) and, on the pull
     road, 40 steps of spike records (5 B) -- per neuron."""
-    per = 12 + 2 + 80 + 8 + 6 + (5 if road == "pull" else 0)
+    per = 12 + 2 + 160 + 8 + 6 + (5 if road == "pull" else 0)
     return per * int(n)
 
 

@@ -284,7 +284,7 @@ class Engine:
         w = np.float32(self.w_scale)
         out = np.zeros((self.batch, self.buf_len, self.n), np.float32)
         for u in range(max(0, T - self.buf_len), T):
-            I = ring[u % L.RING_LEN, :, :self.batch].T
+            I = ring[L.write_slot(u - 1), :, :self.batch].T
             out[:, u % self.buf_len, :] = w * I.astype(np.float32)
         return torch.from_numpy(out)
 
@@ -581,15 +581,15 @@ class Engine:
                 raise ValueError("delay_buffer slot %d is not a multiple of "
                                  "wScale: not a state of this model"
                                  % (u % self.buf_len))
-            ring[u % L.RING_LEN, :, :self.batch] = I.T.astype(np.int32)
+            ring[L.write_slot(u - 1), :, :self.batch] = I.T.astype(np.int32)
         s.ring.copy_(torch.from_numpy(ring.reshape(-1)))
-        s.seed_ring(T % L.RING_LEN)
+        s.seed_ring(L.write_slot(T - 1))
         if s.road == "pull":
             # everything in flight is now in the int32 ring, which the next
             # 20 steps consume; the spike records hold nothing older
             s.rec.zero_()
             s.act.zero_()
-            s.ring_until = T + L.RING_LEN
+            s.ring_until = T + L.DELAY
 
 
 def build_engine(weights=None, neurons=None, silenced=None, params=None,

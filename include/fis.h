@@ -34,9 +34,13 @@
 extern "C" {
 #endif
 
-#define FIS_ABI_VERSION 6
+#define FIS_ABI_VERSION 7
 #define FIS_MAX_WINDOW 20   /* steps per window: delay horizon 19 + 1     */
-#define FIS_RING_LEN 20     /* int32 ring slots: rec(t-19) .. rec(t)       */
+#define FIS_RING_LEN 40     /* int32 ring slots, two delays: the spikes of
+                               step u are summed into slot (u + 21) mod 40,
+                               which step u + 20 reads (slot (t+1) mod 40
+                               for step t); a window's reads and adds never
+                               share a slot                                */
 
 /* Line rule.  Within one kernel, outputs written by different workgroups
  * never share a 128-byte line (on MI355X such plain stores from different
@@ -258,7 +262,7 @@ int fis_pull_resend(const fis_sim* s, const int32_t* idx, int n_idx,
 
 /* Add the input still in flight in the pull road's records (the spikes of
  * steps clock-20 .. clock-1) into `ring` ([FIS_RING_LEN][n * bstride],
- * the push road's layout: spikes of step u land in slot (u+1) mod 20),
+ * the push road's layout: spikes of step u land in slot (u+21) mod 40),
  * using the W^T CSR -- for state views and snapshots. */
 int fis_pull_to_ring(const fis_sim* s, int32_t* ring, void* stream);
 

@@ -76,4 +76,4 @@ def test_workload_generator_and_memory_model():
     assert all(20 <= r.drive.size <= 200 for r in rs)
     assert all(50.0 <= r.rate_hz <= 250.0 for r in rs)
     assert all(r.silenced.size <= 50 for r in rs)
-    assert bytes_per_trial(138_639, "pull") == 113 * 138_639
+    assert bytes_per_trial(138_639, "pull") == 193 * 138_639
