@@ -50,7 +50,8 @@ def timed(fn, sync):
 
 def kernel_times(runner, windows):
     """Average durations (ms) of the drive launch and of the road's
-    launches -- push: update, scatter; pull: the fused window kernel --
+    launches -- push: update, scatter (one fused launch below 64 trials);
+    pull: the fused window kernel --
     over ``windows`` eager windows, bracketed by HIP events on the launch
     stream."""
     import torch
@@ -62,6 +63,8 @@ def kernel_times(runner, windows):
          and s.drive_table(runner.table, W))]
     if s.road == "pull":
         stages.append(("pull", lambda: s.pull(W)))
+    elif s.bstride < 64:
+        stages.append(("update_scatter", lambda: s.update_scatter(W)))
     else:
         stages += [("update", lambda: s.update(W)),
                    ("scatter", lambda: s.scatter())]
@@ -331,9 +334,9 @@ def main():
     kt = head["kt"]
     dom = max((k for k in kt if k != "drive"), key=lambda k: kt[k])
     alg = 0
-    if dom in ("update", "pull"):
+    if dom in ("update", "pull", "update_scatter"):
         alg += BETA * n * B * W
-    if dom in ("scatter", "pull"):
+    if dom in ("scatter", "pull", "update_scatter"):
         # spikes of the last window, from a short raster run
         from flybrain_amd.batch import BatchRunner as _BR
         rr = _BR(net, wl, counts=False, raster_len=W)
@@ -343,7 +346,8 @@ def main():
         alg += scatter_bytes(conn, rows)
         del rr
     achieved = alg / (kt[dom] * 1e-3) / 1e9
-    kname = {"update": "k_update", "pull": "k_pull"}.get(dom) or (
+    kname = {"update": "k_update", "update_scatter": "k_update",
+             "pull": "k_pull"}.get(dom) or (
         "k_scatter_steps" if head["runner"].sim.bstride >= 64 else "k_scatter")
     traffic = profiled_traffic(kname)
     out.update({

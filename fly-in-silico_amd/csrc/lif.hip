@@ -448,9 +448,54 @@ __device__ __forceinline__ uint32_t run_window(const fis_sim& s, Cell& c,
   return fired;
 }
 
+// One event's CSR row, read once and added into every trial of its mask.
+__device__ __forceinline__ void scatter_event(const fis_sim& s,
+                                              const fis_event& ev, int lane) {
+  const int64_t nb = (int64_t)s.n * s.bstride;  // ring slot stride
+  const int bs = s.bstride;
+  const int64_t beg = s.offsets[ev.pre], end = s.offsets[ev.pre + 1];
+  int32_t* base = s.ring + (int64_t)ev.slot * nb + ev.tb;
+  for (int64_t k = beg + lane; k < end; k += kWave) {
+    const int32_t post = s.cols[k];
+    const int32_t val = s.vals[k];
+    int32_t* p = base + (int64_t)post * bs;
+    uint64_t m = ev.mask;
+    while (m) {
+      const int j = __ffsll((long long)m) - 1;
+      m &= m - 1ull;
+      __hip_atomic_fetch_add(p + j, val, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// The CSR rows of the neurons whose lead lane is set, each added into
+// the trials of its `sub` mask in ring slot `slot`, one row after another
+// with the whole wave on each (lead, pre and sub are per lane; the loop is
+// uniform).
+__device__ __forceinline__ void scatter_leads(const fis_sim& s, bool lead,
+                                              int pre, uint64_t sub, int slot,
+                                              int lane) {
+  uint64_t leads = __ballot(lead);
+  asm volatile("s_mov_b64 %0, %1" : "=s"(leads) : "s"(leads));
+  while (leads) {
+    const int l = __ffsll((long long)leads) - 1;
+    leads &= leads - 1ull;
+    fis_event ev;
+    ev.pre = __builtin_amdgcn_readlane(pre, l);
+    ev.slot = (uint16_t)slot;
+    ev.tb = 0;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)sub, l);
+    const uint32_t hi =
+        (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(sub >> 32), l);
+    ev.mask = ((uint64_t)hi << 32) | lo;
+    scatter_event(s, ev, lane);
+  }
+}
+
 // W steps per neuron-trial.  WMAX is the unrolled bound (1 or 20).
 // Index arithmetic is 32-bit: n * bstride < 2^31 is checked on the host.
-template <int WMAX>
+template <int WMAX, bool FUSED>
 __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
   const int bs = s.bstride;
   const int nb = s.n * bs;
@@ -503,6 +548,36 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
   const int kstop = left <= 0 ? 0 : (left >= W ? W : (int)left);
   const uint32_t nspk = __popc(fired_bits & ((1u << kstop) - 1u));
 
+  if constexpr (FUSED) {
+    // below 64 trials the wave adds its own spikes into the ring slots no
+    // step of this window reads (no event records, no scatter launch)
+    if (act) store_cell(s, c, idx, pre, nspk);
+    const uint32_t sent = transmit ? fired_bits : 0u;
+    const bool any_sent = any_lane(sent != 0u);
+    if (s.raster || any_sent) {
+      int rrow = s.raster ? (int)(t0 % s.raster_len) : 0;
+      int slot = (int)((t0 + kDelay + 1) % FIS_RING_LEN);
+#pragma unroll 1
+      for (int k = 0; k < W; ++k) {
+        if (s.raster) {
+          const uint64_t all =
+              group_or(((fired_bits >> k) & 1u) ? (1ull << lane) : 0ull, kWave);
+          s.raster[((int64_t)rrow * nblk + (wave >> 2)) * FIS_LINE_WORDS +
+                   (wave & 3)] = all;
+          rrow = (rrow + 1 == s.raster_len) ? 0 : rrow + 1;
+        }
+        const bool snd = (sent >> k) & 1u;
+        if (any_sent && any_lane(snd)) {
+          // the trials of each neuron (bs lanes from trial 0) that sent
+          const uint64_t sub = group_or(snd ? (1ull << tb) : 0ull, bs);
+          scatter_leads(s, tb == 0 && sub != 0ull, pre, sub, slot, lane);
+        }
+        slot = (slot + 1 == FIS_RING_LEN) ? 0 : slot + 1;
+      }
+    }
+    return;  // the clock moves on in k_tick, after every workgroup read it
+  }
+
   // spikes out, step by step: raster rows, and events (below 64 trials
   // only steps with spikes append any, so a wave that stayed silent for the
   // whole window skips this)
@@ -530,27 +605,6 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
 }
 
 // ------------------------------------------------------------- scatter
-
-// One event's CSR row, read once and added into every trial of its mask.
-__device__ __forceinline__ void scatter_event(const fis_sim& s,
-                                              const fis_event& ev, int lane) {
-  const int64_t nb = (int64_t)s.n * s.bstride;  // ring slot stride
-  const int bs = s.bstride;
-  const int64_t beg = s.offsets[ev.pre], end = s.offsets[ev.pre + 1];
-  int32_t* base = s.ring + (int64_t)ev.slot * nb + ev.tb;
-  for (int64_t k = beg + lane; k < end; k += kWave) {
-    const int32_t post = s.cols[k];
-    const int32_t val = s.vals[k];
-    int32_t* p = base + (int64_t)post * bs;
-    uint64_t m = ev.mask;
-    while (m) {
-      const int j = __ffsll((long long)m) - 1;
-      m &= m - 1ull;
-      __hip_atomic_fetch_add(p + j, val, __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
-}
 
 // bstride < 64: one wave per update-wave segment (grid-stride) over the
 // compacted events of the window.
@@ -1292,11 +1346,26 @@ int fis_update(const fis_sim* s, int W, void* stream) {
   if (W < 1 || W > FIS_MAX_WINDOW) return (int)hipErrorInvalidValue;
   const int64_t nb = (int64_t)s->n * s->bstride;
   if (W == 1)
-    hipLaunchKernelGGL(k_update<1>, dim3(grid_for(nb)), dim3(kBlock), 0,
-                       (hipStream_t)stream, *s, W);
+    hipLaunchKernelGGL((k_update<1, false>), dim3(grid_for(nb)), dim3(kBlock),
+                       0, (hipStream_t)stream, *s, W);
   else
-    hipLaunchKernelGGL(k_update<FIS_MAX_WINDOW>, dim3(grid_for(nb)),
+    hipLaunchKernelGGL((k_update<FIS_MAX_WINDOW, false>), dim3(grid_for(nb)),
                        dim3(kBlock), 0, (hipStream_t)stream, *s, W);
+  return (int)hipGetLastError();
+}
+int fis_update_scatter(const fis_sim* s, int W, void* stream) {
+  if (int rc = check_sim(s)) return rc;
+  if (W < 1 || W > FIS_MAX_WINDOW || s->bstride >= kWave)
+    return (int)hipErrorInvalidValue;
+  const int64_t nb = (int64_t)s->n * s->bstride;
+  if (W == 1)
+    hipLaunchKernelGGL((k_update<1, true>), dim3(grid_for(nb)), dim3(kBlock),
+                       0, (hipStream_t)stream, *s, W);
+  else
+    hipLaunchKernelGGL((k_update<FIS_MAX_WINDOW, true>), dim3(grid_for(nb)),
+                       dim3(kBlock), 0, (hipStream_t)stream, *s, W);
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, (hipStream_t)stream,
+                     s->clock, W);
   return (int)hipGetLastError();
 }
 

@@ -140,6 +140,7 @@ def lib():
         "fis_drive_tape": ([sims, P, P, I64, I, P], I),
         "fis_update": ([sims, I, P], I),
         "fis_scatter": ([sims, I, P], I),
+        "fis_update_scatter": ([sims, I, P], I),
         "fis_seed_ring": ([sims, I, P], I),
         "fis_resend": ([sims, P, I, I, P], I),
         "fis_pull_window": ([sims, I, I, P], I),

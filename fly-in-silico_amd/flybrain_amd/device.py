@@ -517,9 +517,19 @@ class DeviceSim:
         """The road's launches of one window after its drive."""
         if self.road == "pull":
             self.pull(W)
+        elif self.bstride < 64:
+            self.update_scatter(W)
         else:
             self.update(W)
             self.scatter()
+
+    def update_scatter(self, W):
+        """Below 64 trials: the window's update and scatter as one launch
+        (clock += W included)."""
+        self._push_only("update_scatter")
+        L.check(self.lib.fis_update_scatter(self._sp, W, _stream()),
+                "fis_update_scatter")
+        self.t += W
 
     def window(self, W, drive=None):
         """One window: drive (a DriveTable, Tape or rates tensor), update,

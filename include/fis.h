@@ -228,6 +228,12 @@ int fis_update(const fis_sim* s, int W, void* stream);
  * exact), then clock += W. */
 int fis_scatter(const fis_sim* s, int W, void* stream);
 
+/* fis_update + fis_scatter without the event records, bstride < 64 only:
+ * each wave adds its window's spikes straight into the ring slots that no
+ * step of the window reads (Engine._core + the SpMV, sim.py:906-953); a
+ * one-thread launch then advances the clock by W. */
+int fis_update_scatter(const fis_sim* s, int W, void* stream);
+
 /* Scatter the current spike state (st bit 7, not FIS_OUT_OFF) into ring
  * slot `slot` -- used after a restore to rebuild the input in flight. */
 int fis_seed_ring(const fis_sim* s, int slot, void* stream);
