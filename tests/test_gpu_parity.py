@@ -295,6 +295,54 @@ def test_ev_spmv_twin_matches_dense():
     assert lib.fis_ev_spmv_abi() == 3
 
 
+@pytest.mark.parametrize("batch", [1, 3, 32])
+def test_one_launch_push_window_equals_two(batch):
+    """Below 64 trials the push road runs update and scatter as one launch
+    (fis_update_scatter: spikes added straight into ring slots the window
+    does not read).  The two-launch form (fis_update + fis_scatter, event
+    records) gives the same raster, state and ring window by window, over
+    windows of 1..20 steps, and both equal the oracle."""
+    conn = C.synthetic(512, None, mean_out=40, count_exp=1.2, n_groups=4,
+                       seed=29)
+    n = conn.n
+    rng = np.random.default_rng(batch + 11)
+    seeds = rng.integers(0, 2**63, size=batch, dtype=np.uint64)
+    drv = [rng.choice(n, rng.integers(5, 40), replace=False)
+           for _ in range(batch)]
+    rate = rng.uniform(100, 900, size=batch).astype(np.float32)
+    net = DeviceNet(conn)
+    ne = np.concatenate(drv)
+    tr = np.concatenate([np.full(len(d), b) for b, d in enumerate(drv)])
+    sims = []
+    for _ in range(2):
+        sim = DeviceSim(net, batch, seeds=seeds, raster_len=40, road="push")
+        sim.exempt_pairs(ne, tr)
+        sims.append((sim, DriveTable(sim, ne, tr, tr, rate)))
+    (one, t_one), (two, t_two) = sims
+    ora = O.OracleEngine(conn, batch, seeds=seeds, no_refractory_trials=drv)
+    rates = np.zeros((batch, n), np.float32)
+    for b, d in enumerate(drv):
+        rates[b, d] = rate[b]
+    a = 0
+    for W in (20, 7, 13, 1, 19, 20, 20):
+        one.drive_table(t_one, W)
+        one.update_scatter(W)
+        two.drive_table(t_two, W)
+        two.update(W)
+        two.scatter()
+        r_one = one.read_raster(a, a + W)
+        assert np.array_equal(r_one, two.read_raster(a, a + W)), \
+            "rasters differ in the window at %d" % a
+        for k in range(W):
+            ora.step_rates(rates)
+            assert np.array_equal(r_one[k], ora.spikes.astype(bool)), \
+                "raster differs from the oracle at step %d" % (a + k)
+        compare_state(one, ora, "one launch, window end %d" % (a + W - 1))
+        compare_state(two, ora, "two launches, window end %d" % (a + W - 1))
+        assert torch.equal(one.ring, two.ring), "rings differ at %d" % a
+        a += W
+
+
 @pytest.mark.parametrize("road,batch", [("push", 64), ("push", 3),
                                         ("push", 130), ("pull", 64),
                                         ("pull", 200)])
