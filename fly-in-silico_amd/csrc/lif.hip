@@ -722,13 +722,14 @@ __device__ __forceinline__ uint64_t rot40(uint64_t x, int r) {
   return r == 0 ? x : (((x << r) | (x >> (kRec - r))) & kMask40);
 }
 
-// acc[b] += val for every set bit b of the 64-trial word x
+// acc[b * stride] += val for every set bit b of the 64-trial word x
+template <int stride>
 __device__ __forceinline__ void add_bits(int32_t* acc, uint64_t x,
                                          int32_t val) {
   while (x) {
     const int b = __ffsll((long long)x) - 1;
     x &= x - 1ull;
-    atomicAdd(acc + b, val);
+    atomicAdd(acc + b * stride, val);
   }
 }
 
@@ -810,7 +811,10 @@ __attribute__((amdgpu_waves_per_eu(8, 8)))
 void k_pull(fis_sim s, int W,
                                                int ring_steps, int n_tiles,
                                                int per_xcd) {
-  __shared__ int32_t acc[WMAX * TILE];  // WMAX x 4 B per lane, nothing else
+  // acc[trial][step]: a lane's WMAX cells are contiguous (cleared and read
+  // back with 16-byte LDS accesses); nothing else lives in LDS
+  static_assert(WMAX % 4 == 0, "acc rows are cleared 4 cells at a time");
+  __shared__ __align__(16) int32_t acc[TILE * WMAX];
 
   constexpr int bs = BS;                        // trial stride (== s.bstride)
   constexpr int tpn = TILE < BS ? TILE : BS;    // trials of a neuron per tile
@@ -835,7 +839,9 @@ void k_pull(fis_sim s, int W,
   const int pstride = kRec * words;                  // words per neuron
   const uint64_t wmask = (1ull << W) - 1;
 
-  for (int j = 0; j < W; ++j) acc[j * TILE + tid] = 0;
+#pragma unroll
+  for (int j = 0; j < WMAX; j += 4)
+    *(int4*)(acc + tid * WMAX + j) = make_int4(0, 0, 0, 0);
   // the tile's in-edge offsets: uniform, held in scalar registers
   int off[P + 1];
 #pragma unroll
@@ -951,10 +957,10 @@ void k_pull(fis_sim s, int W,
         }
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
-          int32_t* at = acc + pr[r].step * TILE + pr[r].pos * tpn;
+          int32_t* at = acc + pr[r].pos * tpn * WMAX + pr[r].step;
 #pragma unroll
           for (int q = 0; q < wpt; ++q)
-            add_bits(at + q * kWave, xw[r][q], pr[r].val);
+            add_bits<WMAX>(at + q * kWave * WMAX, xw[r][q], pr[r].val);
         }
       }
       e = en;
@@ -978,7 +984,7 @@ void k_pull(fis_sim s, int W,
   // it waits for memory)
   int32_t in[WMAX];
 #pragma unroll
-  for (int k = 0; k < WMAX; ++k) in[k] = k < W ? acc[k * TILE + tid] : 0;
+  for (int k = 0; k < WMAX; ++k) in[k] = acc[tid * WMAX + k];  // 0 past W
   if (ring_steps > 0 && act) {  // input carried over from a restore
     int32_t* rp = s.ring + idx;
     int rslot = (int)((t0 + 1) % FIS_RING_LEN);
@@ -1002,30 +1008,30 @@ void k_pull(fis_sim s, int W,
   if (act) store_cell(s, c, idx, pre, nspk);
 
   // The wave's record words of the window (and raster rows): lane k < W
-  // owns step k's.  This wave's 64 cells of every acc row are its own and
-  // already read; row k's first two 8-byte cells collect step k's words
-  // (LDS atomics of the lanes that fired, few), cell 4 of row 0 the steps
-  // at which the wave sent anything.
+  // owns step k's.  This wave's 64 x WMAX cells are its own and already
+  // read; cells 4k .. 4k+3 collect step k's two words (LDS atomics of the
+  // lanes that fired, few), cell 4 WMAX the steps at which the wave sent
+  // anything.
   {
     const uint32_t sent = transmit ? fired_bits : 0u;
-    int32_t* mine = acc + (tid & ~(kWave - 1));  // this wave's cells, row 0
-    uint64_t* cell = (uint64_t*)(mine + (lane < W ? lane : 0) * TILE);
+    int32_t* mine = acc + (tid & ~(kWave - 1)) * WMAX;  // this wave's cells
+    uint64_t* cell = (uint64_t*)(mine + 4 * (lane < W ? lane : 0));
     if (lane < W) {
       cell[0] = 0ull;
       cell[1] = 0ull;
     }
-    if (lane == W) mine[4] = 0;  // (row 0: its words sit in cells 0..3)
+    if (lane == W) mine[4 * WMAX] = 0;
     __builtin_amdgcn_wave_barrier();
     const uint64_t me = 1ull << lane;
     for (uint32_t x = sent; x; x &= x - 1u)
-      atomicOr((uint64_t*)(mine + (__ffs(x) - 1) * TILE), me);
+      atomicOr((uint64_t*)(mine + 4 * (__ffs(x) - 1)), me);
     if (s.raster)
       for (uint32_t x = fired_bits; x; x &= x - 1u)
-        atomicOr((uint64_t*)(mine + (__ffs(x) - 1) * TILE) + 1, me);
+        atomicOr((uint64_t*)(mine + 4 * (__ffs(x) - 1)) + 1, me);
     __builtin_amdgcn_wave_barrier();
     if (lane < W) {
       const uint64_t word = cell[0];
-      if (word != 0ull) atomicOr(mine + 4, 1 << lane);
+      if (word != 0ull) atomicOr(mine + 4 * WMAX, 1 << lane);
       // `act` is uniform over a wave (n * bstride is a multiple of 64): the
       // waves past the last neuron of the last tile store nothing
       if (act) {
@@ -1046,7 +1052,7 @@ void k_pull(fis_sim s, int W,
   if (tid < np) {
     uint32_t bits32 = 0;
     for (int q = 0; q < wpt; ++q)
-      bits32 |= (uint32_t)acc[tid * tpn + q * kWave + 4];
+      bits32 |= (uint32_t)acc[(tid * tpn + q * kWave) * WMAX + 4 * WMAX];
     uint64_t* a = s.act + (int64_t)s.order[p0 + tid] * parts + part;
     __hip_atomic_fetch_and(a, ~rot40(wmask, wbase), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
