@@ -385,7 +385,10 @@ __device__ __forceinline__ Stim make_stim(const fis_sim& s) {
 __device__ __forceinline__ bool lif_step(const fis_sim& s, Cell& c, int32_t I,
                                          bool kick, const Stim& stim) {
   // refractory counter: r = (r + 1) * (1 - s), saturating; gate = r >= R
-  c.r = c.sp ? 0 : (c.r < FIS_ST_RMAX ? c.r + 1 : FIS_ST_RMAX);
+  // (it saturates at FIS_ST_RMAX only when stored: within a window it
+  // stays below FIS_ST_RMAX + FIS_MAX_WINDOW, and the gate compares it with
+  // refrac_steps <= 126 or INT_MAX)
+  c.r = c.sp ? 0 : c.r + 1;
 
   // conductance with the delayed input (sim.py:927-929):
   // g * a_syn + (w_scale * I) * m_in * gate
@@ -413,7 +416,8 @@ __device__ __forceinline__ void store_cell(const fis_sim& s, const Cell& c,
                                            int idx, int pre, uint32_t nspk) {
   s.v[idx] = c.v;
   s.g[idx] = c.g;
-  s.st[idx] = (uint8_t)((c.sp ? FIS_ST_SPIKE : 0u) | (uint32_t)c.r);
+  s.st[idx] = (uint8_t)((c.sp ? FIS_ST_SPIKE : 0u) |
+                        (uint32_t)min(c.r, (int)FIS_ST_RMAX));
   if (s.spikes_out) s.spikes_out[idx] = c.sp ? 1.0f : 0.0f;
   if (nspk) {
     if (s.counts) s.counts[idx] += nspk;
@@ -1226,6 +1230,9 @@ int check_sim(const fis_sim* s) {
       !s->clock || !s->seeds || !s->trial_ids)
     return (int)hipErrorInvalidValue;
   if (s->raster && s->raster_len < FIS_MAX_WINDOW)
+    return (int)hipErrorInvalidValue;
+  // the counter saturates at FIS_ST_RMAX = 127 between windows
+  if (s->refrac_steps < 0 || s->refrac_steps >= FIS_ST_RMAX)
     return (int)hipErrorInvalidValue;
   if (s->drive_group && (!s->drive_rate || s->drive_n_groups <= 0 ||
                          s->drive_n_seg <= 0 || s->drive_seg_steps <= 0))
