@@ -258,7 +258,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0,
                     help="Poisson arrival rate of requests per second")
     ap.add_argument("--serve-slots", type=int, default=256)
-    ap.add_argument("--serve-shard", type=int, default=32,
+    ap.add_argument("--serve-shard", type=int, default=16,
                     help="slots per shard (one DeviceSim each)")
     ap.add_argument("--road", choices=("auto", "push", "pull"), default="auto",
                     help="auto: pull road for 64..1024 trials, push below")

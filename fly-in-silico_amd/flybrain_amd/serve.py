@@ -189,13 +189,15 @@ class Server:
 
     Each shard is its own DeviceSim; a window launches only the shards
     that hold a live request, so the GPU time of a window follows the live
-    load rather than the capacity.  The default shard of 64 trials is the
-    pull road's smallest stride (one 64-bit trial word).
+    load rather than the capacity.  The default shard of 16 trials (push
+    road, one launch per window) measured best on the full connectome:
+    p50 33 ms and p99 73 ms at 200 requests/s, against 60 / 155 ms with 32
+    and 115 / 266 ms with 64 (pull road).
     """
 
     def __init__(self, net: DeviceNet, slots=256, window=20, road="auto",
                  hbm_fraction=0.5, max_drive=None, graph_windows=0,
-                 shard=64):
+                 shard=16):
         self.net = net
         cap = self.capacity(net, road, hbm_fraction)
         self.slots = int(min(slots, cap))
