@@ -154,7 +154,8 @@ def profiled_traffic(kernel):
             if len(parts) < 5 or parts[-3] not in ("FETCH_SIZE", "WRITE_SIZE"):
                 continue
             head = line[:60]
-            if kernel + "(" not in head and kernel + "<" not in head:
+            if (kernel + "(" not in head and kernel + "<" not in head
+                    and not (kernel.endswith(">") and kernel in head)):
                 continue
             grid, name, val = int(parts[-4]), parts[-3], float(parts[-1])
             if grid >= best.get(name, (0, 0))[0]:
@@ -260,8 +261,12 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=256)
     ap.add_argument("--serve-shard", type=int, default=16,
                     help="slots per shard (one DeviceSim each)")
-    ap.add_argument("--road", choices=("auto", "push", "pull"), default="auto",
-                    help="auto: pull road for 64..1024 trials, push below")
+    ap.add_argument("--road", choices=("tuned", "auto", "push", "pull"),
+                    default="tuned",
+                    help="tuned: the (road, pull workgroup shape) the "
+                         "autotuner measures fastest for each workload "
+                         "(cached); auto: pull road for 64..1024 trials, "
+                         "push below")
     ap.add_argument("--no-graph", action="store_true",
                     help="eager launches (for per-dispatch counter runs)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -303,6 +308,14 @@ def main():
         else:
             wl = WL.plain_b1(conn.n, seed=0, steps=args.warmup + args.b1_steps)
             steps = args.b1_steps
+        tuned = None
+        if args.road == "tuned":
+            from flybrain_amd.autotune import tune
+            t1 = time.perf_counter()
+            tuned = tune(net, wl)
+            log("%s: tuned %s in %.1f s, ms per window %s"
+                % (name, tuned["best"], time.perf_counter() - t1,
+                   tuned["times"]))
         r = BatchRunner(net, wl, counts=True, road=args.road)
         if not args.no_graph:
             r.capture(args.graph_windows)
@@ -312,7 +325,7 @@ def main():
         spikes = int(r.counts().sum())
         kt = kernel_times(r, 20)
         res[name] = dict(wl=wl, runner=r, wall=wall, steps=steps,
-                         spikes=spikes, kt=kt,
+                         spikes=spikes, kt=kt, tuned=tuned,
                          tps=wl.batch * steps * 1e-4 / wall)
         log("%s: B=%d %d steps %.4f s  %.3f us/step  %.2f trial-s/s  "
             "spikes/trial-step %.2f  kernels(ms) %s"
@@ -349,6 +362,9 @@ def main():
     kname = {"update": "k_update", "update_scatter": "k_update",
              "pull": "k_pull"}.get(dom) or (
         "k_scatter_steps" if head["runner"].sim.bstride >= 64 else "k_scatter")
+    if dom == "pull":  # the instantiation that ran: window, stride, tile
+        hs = head["runner"].sim
+        kname = "k_pull<%d, %d, %d>" % (20, hs.bstride, hs.pull_tile)
     traffic = profiled_traffic(kname)
     out.update({
         "value": round(head["tps"], 3),
@@ -357,6 +373,11 @@ def main():
         "config": {"workload": wl.name, "neurons": n,
                    "edges": conn.n_edges, "batch": B,
                    "road": head["runner"].sim.road,
+                   "pull_tile": (head["runner"].sim.pull_tile
+                                 if head["runner"].sim.road == "pull"
+                                 else None),
+                   "autotuned": (head["tuned"]["times"] if head["tuned"]
+                                 else None),
                    "window_steps": W, "graph_windows": args.graph_windows,
                    "connectome": conn.fingerprint()},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),

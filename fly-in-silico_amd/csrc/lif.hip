@@ -788,22 +788,22 @@ __device__ __forceinline__ Pair deal_pair(int pi, int excl, uint32_t m,
 // typical 100-odd in-edges and still give each wave two rounds to overlap.
 constexpr int kPullShare = 2;
 
-// Neuron-trials per workgroup for a trial stride.  256 lanes (4 waves,
-// 20 KB of accumulators) measured fastest at 256 trials: smaller
-// workgroups wait less at the barrier between gather and update for their
-// slowest wave, and 8 of them fit a CU.
-template <int BS>
-constexpr int pull_tile() {
-  return FIS_PULL_TILE;
+// Neuron-trials per workgroup (fis_sim.pull_tile; 0 = FIS_PULL_TILE).  256
+// lanes (4 waves, 20 KB of accumulators) measured fastest at 256 trials:
+// smaller workgroups wait less at the barrier between gather and update for
+// their slowest wave, and 8 of them fit a CU; 128 and 512 are compiled too
+// and picked by measurement where they win.
+__host__ __device__ inline int pull_tile_of(const fis_sim& s) {
+  return s.pull_tile ? s.pull_tile : FIS_PULL_TILE;
 }
 
 // trials of one neuron in one tile, and tiles per neuron (act words per
-// neuron), for a trial stride of the pull road
-__host__ __device__ constexpr int pull_trials_per_part(int bs) {
-  return bs < FIS_PULL_TILE ? bs : FIS_PULL_TILE;
+// neuron), for a trial stride and tile of the pull road
+__host__ __device__ inline int pull_trials_per_part(int bs, int tile) {
+  return bs < tile ? bs : tile;
 }
-__host__ __device__ constexpr int pull_parts(int bs) {
-  return bs / pull_trials_per_part(bs);
+__host__ __device__ inline int pull_parts(int bs, int tile) {
+  return bs / pull_trials_per_part(bs, tile);
 }
 
 // Scalar registers capped at 80: measured on MI355X, a workgroup of this
@@ -1093,8 +1093,9 @@ __global__ __launch_bounds__(kBlock) void k_pull_resend(fis_sim s,
     __hip_atomic_fetch_and(w, ~bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   } else {
     __hip_atomic_fetch_or(w, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_or(s.act + (int64_t)pre * pull_parts(s.bstride) +
-                              b / pull_trials_per_part(s.bstride),
+    const int tile = pull_tile_of(s);
+    __hip_atomic_fetch_or(s.act + (int64_t)pre * pull_parts(s.bstride, tile) +
+                              b / pull_trials_per_part(s.bstride, tile),
                           1ull << slot, __ATOMIC_RELAXED,
                           __HIP_MEMORY_SCOPE_AGENT);
   }
@@ -1114,7 +1115,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_to_ring(fis_sim s,
   const int words = s.bstride >> 6;
   const int64_t nb = (int64_t)s.n * s.bstride;
   int32_t* base = ring + (int64_t)((u + kDelay + 1) % FIS_RING_LEN) * nb;
-  const int parts = pull_parts(s.bstride);
+  const int parts = pull_parts(s.bstride, pull_tile_of(s));
   const int wpp = words / parts;  // words per part
   for (int w = 0; w < words; ++w) {
     if (!((s.act[(int64_t)pre * parts + w / wpp] >> slot) & 1ull)) continue;
@@ -1251,6 +1252,9 @@ int check_pull(const fis_sim* s) {
   if (!s->in_offsets || !s->in_cols || !s->in_vals || !s->rec || !s->act ||
       !s->order)
     return (int)hipErrorInvalidValue;
+  if (s->pull_tile != 0 && s->pull_tile != 128 && s->pull_tile != 256 &&
+      s->pull_tile != 512)
+    return (int)hipErrorInvalidValue;
   // the pull road draws a drive table in its own launch: the window kernel
   // is bound by vector-instruction issue, and Philox on the few driven
   // lanes of a wave would cost it more than a separate, fully occupied
@@ -1282,6 +1286,7 @@ int64_t fis_struct_layout(int what) {
     case 6: return (int64_t)sizeof(fis_drive_entry);
     case 7: return (int64_t)offsetof(fis_sim, in_offsets);
     case 8: return (int64_t)offsetof(fis_sim, t_origin);
+    case 9: return (int64_t)offsetof(fis_sim, pull_tile);
     default: return -1;
   }
 }
@@ -1425,19 +1430,19 @@ int fis_pull_window(const fis_sim* s, int W, int ring_steps, void* stream) {
   if (int rc = check_pull(s)) return rc;
   if (W < 1 || W > FIS_MAX_WINDOW || ring_steps < 0)
     return (int)hipErrorInvalidValue;
-  auto kern = k_pull<FIS_MAX_WINDOW, 256, pull_tile<256>()>;
-  int tile = 0;
-  switch (s->bstride) {
-#define FIS_PULL_CASE(B)                                  \
-  case B:                                                 \
-    kern = k_pull<FIS_MAX_WINDOW, B, pull_tile<B>()>;     \
-    tile = pull_tile<B>();                                \
+  const int tile = pull_tile_of(*s);
+  auto kern = k_pull<FIS_MAX_WINDOW, 256, 256>;
+  switch (s->bstride * 4 + tile / 128) {  // tile / 128 is 1, 2 or 4
+#define FIS_PULL_CASE(B, T)                  \
+  case B * 4 + T / 128:                      \
+    kern = k_pull<FIS_MAX_WINDOW, B, T>;     \
     break;
-    FIS_PULL_CASE(64)
-    FIS_PULL_CASE(128)
-    FIS_PULL_CASE(256)
-    FIS_PULL_CASE(512)
-    FIS_PULL_CASE(1024)
+    FIS_PULL_CASE(64, 128) FIS_PULL_CASE(64, 256) FIS_PULL_CASE(64, 512)
+    FIS_PULL_CASE(128, 128) FIS_PULL_CASE(128, 256) FIS_PULL_CASE(128, 512)
+    FIS_PULL_CASE(256, 128) FIS_PULL_CASE(256, 256) FIS_PULL_CASE(256, 512)
+    FIS_PULL_CASE(512, 128) FIS_PULL_CASE(512, 256) FIS_PULL_CASE(512, 512)
+    FIS_PULL_CASE(1024, 128) FIS_PULL_CASE(1024, 256)
+    FIS_PULL_CASE(1024, 512)
 #undef FIS_PULL_CASE
     default: return (int)hipErrorInvalidValue;
   }

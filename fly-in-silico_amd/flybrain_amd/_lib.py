@@ -20,7 +20,7 @@ CSRC = PKG.parent / "csrc"
 SOURCES = [CSRC / "lif.hip"]
 HEADER = ROOT / "include" / "fis.h"
 LIB_PATH = PKG / "libfis.so"
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 MAX_WINDOW = 20
 RING_LEN = 40  # int32 ring slots (two delays)
@@ -38,6 +38,7 @@ def read_slot(t):
 FIRE_OFF, OUT_OFF, IN_OFF, HOLD, NOREFRAC = 0x01, 0x02, 0x04, 0x08, 0x10
 BLOCK_WAVES, LINE_WORDS, LINE_INTS, EVENT_ALIGN = 4, 16, 32, 8
 REC_SLOTS, PULL_TILE, PULL_MAX = 40, 256, 1024
+PULL_TILES = (128, 256, 512)  # pull-road workgroup shapes compiled
 ST_SPIKE, ST_RMAX = 0x80, 127
 
 HIPCC_FLAGS = [
@@ -99,6 +100,7 @@ class FisSim(ctypes.Structure):
         ("drive_group", P), ("drive_rate", P),
         ("drive_n_groups", ctypes.c_int32), ("drive_n_seg", ctypes.c_int32),
         ("drive_seg_steps", ctypes.c_int32),
+        ("pull_tile", ctypes.c_int32),
     ]
 
 

@@ -1,0 +1,111 @@
+"""Trial-block shape picked by measurement.
+
+A batch of trials runs on one of two roads (push: event scatter into an
+int32 ring; pull: a fused gather + update per window, 64..1024 trials) and,
+on the pull road, with one of the compiled workgroup shapes
+(``_lib.PULL_TILES`` neuron-trials per workgroup).  Which is fastest
+depends on the batch, the connectome and how active it is, so
+:func:`tune` times every candidate on the workload itself -- a short
+warm-up to reach its working activity, then a few eager windows bracketed
+by device events -- and keeps the fastest.  Results are cached in a JSON
+file (``$FLYBRAIN_TUNE_CACHE``, default ``~/.cache/flybrain_amd/tune.json``)
+keyed by device, connectome fingerprint, workload and batch; the cache is
+only an optimisation (an unwritable location is skipped).
+"""
+
+from __future__ import annotations
+
+import json
+import os
+from pathlib import Path
+
+from . import _lib as L
+from .device import PULL_MIN_BATCH
+
+
+def cache_path():
+    p = os.environ.get("FLYBRAIN_TUNE_CACHE")
+    return Path(p) if p else Path.home() / ".cache" / "flybrain_amd" / "tune.json"
+
+
+def candidates(batch):
+    """(road, pull_tile) pairs that can run ``batch`` trials."""
+    out = [("push", None)]
+    if PULL_MIN_BATCH <= batch <= L.PULL_MAX:
+        out += [("pull", t) for t in L.PULL_TILES]
+    return out
+
+
+def cache_key(net, wl):
+    import torch
+    dev = torch.cuda.get_device_name(net.device) if net.device.type == "cuda" \
+        else str(net.device)
+    return "%s|%s|%s|%d" % (dev, net.conn.fingerprint(), wl.name, wl.batch)
+
+
+def _load(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(path, table):
+    try:
+        path.parent.mkdir(parents=True, exist_ok=True)
+        tmp = path.with_suffix(".tmp")
+        with open(tmp, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass
+
+
+def time_candidate(net, wl, road, pull_tile, warmup_steps=200, windows=10):
+    """Mean wall time of one 20-step window (ms) of ``wl`` on (road,
+    pull_tile), eager launches, after ``warmup_steps`` steps."""
+    import torch
+
+    from .batch import BatchRunner
+
+    r = BatchRunner(net, wl, counts=False, road=road, pull_tile=pull_tile)
+    r.advance(warmup_steps, use_graph=False)
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    r.advance(windows * r.window, use_graph=False)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / windows
+    del r
+    return ms
+
+
+def tune(net, wl, cache=True, timer=None, **kw):
+    """Pick the fastest (road, pull_tile) for ``wl`` on ``net``.
+
+    Returns ``{"best": {"road", "pull_tile", "ms_per_window"}, "times":
+    {"road/tile": ms}}``.  ``timer(net, wl, road, tile) -> ms`` replaces
+    the device measurement (tests); ``kw`` goes to :func:`time_candidate`.
+    """
+    path = cache_path()
+    key = cache_key(net, wl)
+    table = _load(path) if cache else {}
+    if key in table:
+        return table[key]
+    timer = timer or (lambda n, w, road, tile: time_candidate(n, w, road,
+                                                               tile, **kw))
+    times = {}
+    best = None
+    for road, tile in candidates(wl.batch):
+        ms = float(timer(net, wl, road, tile))
+        times["%s/%s" % (road, tile or "-")] = round(ms, 4)
+        if best is None or ms < best["ms_per_window"]:
+            best = {"road": road, "pull_tile": tile, "ms_per_window": ms}
+    result = {"best": best, "times": times}
+    if cache:
+        table[key] = result
+        _store(path, table)
+    return result

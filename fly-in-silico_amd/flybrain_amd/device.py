@@ -138,9 +138,15 @@ class DeviceSim:
 
     def __init__(self, net: DeviceNet, batch: int, params=None, seeds=None,
                  trial_ids=None, no_refractory=None, spikes_out=False,
-                 counts=False, pop_counts=False, raster_len=0, road="push"):
+                 counts=False, pop_counts=False, raster_len=0, road="push",
+                 pull_tile=None):
         self.lib = L.lib()
         self.road = pick_road(int(batch), road)
+        # pull road: neuron-trials per workgroup (fixed for the simulation:
+        # the act words are laid out by it)
+        self.pull_tile = int(pull_tile or L.PULL_TILE)
+        if self.pull_tile not in L.PULL_TILES:
+            raise ValueError("pull_tile must be one of %s" % (L.PULL_TILES,))
         if net.device.type != "cuda":
             raise L.KernelUnavailable("the engine runs on a GPU only; got "
                                       "device %r" % (net.device,))
@@ -207,7 +213,7 @@ class DeviceSim:
                 net.pull_layout(bs)
             self.rec = torch.zeros(L.REC_SLOTS * n * (bs // 64),
                                    dtype=torch.int64, **z)
-            self.act = torch.zeros(n * max(1, bs // L.PULL_TILE),
+            self.act = torch.zeros(n * max(1, bs // self.pull_tile),
                                    dtype=torch.int64, **z)
         else:
             self.order = self.in_offsets = self.in_cols = self.in_vals = None
@@ -261,6 +267,7 @@ class DeviceSim:
             setattr(s, name, t.data_ptr() if t is not None else None)
         dims = getattr(self, "_drive_dims", (0, 0, 0))
         s.drive_n_groups, s.drive_n_seg, s.drive_seg_steps = dims
+        s.pull_tile = self.pull_tile
         self.struct = s
         self._sp = ctypes.byref(s)
         self.version = getattr(self, "version", 0) + 1

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define FIS_ABI_VERSION 7
+#define FIS_ABI_VERSION 8
 #define FIS_MAX_WINDOW 20   /* steps per window: delay horizon 19 + 1     */
 #define FIS_RING_LEN 40     /* int32 ring slots, two delays: the spikes of
                                step u are summed into slot (u + 21) mod 40,
@@ -135,12 +135,12 @@ typedef struct {
    * spike records of the last 40 steps.  rec word
    * [pre][t mod 40][w] bit j = trial 64 w + j of `pre` transmitted a spike
    * at step t; act[pre][k] bit (t mod 40) = some trial of `pre` among
-   * trials k * FIS_PULL_TILE .. (k + 1) * FIS_PULL_TILE - 1 did.         */
+   * trials k * pull_tile .. (k + 1) * pull_tile - 1 did.                  */
   const int32_t* in_offsets; /* [n+1], by position (see `order`)       */
   const int32_t* in_cols;    /* [E] presynaptic neuron                 */
   const int16_t* in_vals;    /* [E] signed synapse count               */
   uint64_t* rec;             /* [n][40][bstride / 64]                  */
-  uint64_t* act;             /* [n][max(1, bstride / FIS_PULL_TILE)]   */
+  uint64_t* act;             /* [n][max(1, bstride / pull_tile)]       */
   /* the order in which workgroups take the neurons: position k holds
    * neuron order[k] (a permutation of 0..n-1 that moves aligned units of
    * max(1, 256 / bstride) neurons, e.g. sorted by neuropil group, so that
@@ -172,6 +172,11 @@ typedef struct {
   int32_t drive_n_groups;
   int32_t drive_n_seg;
   int32_t drive_seg_steps;
+  /* pull road: neuron-trials per workgroup, 128, 256 or 512 (0: the
+   * default FIS_PULL_TILE).  Fixed for a simulation's life: it sets the
+   * layout of `act` ([n][max(1, bstride / pull_tile)] words).  Chosen by
+   * measurement (flybrain_amd.autotune). */
+  int32_t pull_tile;
 } fis_sim;
 
 /* Drive table entry: neuron-trial `idx` is driven at rate

@@ -1,0 +1,63 @@
+"""The trial-block autotuner's host logic: candidates, choice, cache."""
+
+from __future__ import annotations
+
+import json
+from types import SimpleNamespace
+
+import torch
+
+from flybrain_amd import _lib as L
+from flybrain_amd import autotune as T
+
+
+def fake(batch, fp="c0ffee"):
+    net = SimpleNamespace(device=torch.device("cpu"),
+                          conn=SimpleNamespace(fingerprint=lambda: fp))
+    wl = SimpleNamespace(name="w", batch=batch)
+    return net, wl
+
+
+def test_candidates_follow_the_roads():
+    assert T.candidates(1) == [("push", None)]
+    assert T.candidates(63) == [("push", None)]
+    got = T.candidates(256)
+    assert got[0] == ("push", None)
+    assert [t for r, t in got if r == "pull"] == list(L.PULL_TILES)
+    assert T.candidates(L.PULL_MAX + 1) == [("push", None)]
+
+
+def test_tune_picks_the_fastest_and_caches(tmp_path, monkeypatch):
+    monkeypatch.setenv("FLYBRAIN_TUNE_CACHE", str(tmp_path / "t.json"))
+    net, wl = fake(256)
+    cost = {("push", None): 8.0, ("pull", 128): 1.2, ("pull", 256): 1.1,
+            ("pull", 512): 1.4}
+    calls = []
+
+    def timer(n, w, road, tile):
+        calls.append((road, tile))
+        return cost[(road, tile)]
+
+    res = T.tune(net, wl, timer=timer)
+    assert res["best"]["road"] == "pull" and res["best"]["pull_tile"] == 256
+    assert res["times"]["push/-"] == 8.0 and len(res["times"]) == 4
+    assert sorted(calls, key=str) == sorted(cost, key=str)
+    # cached: no second measurement, same answer
+    again = T.tune(net, wl, timer=lambda *a: 1 / 0)
+    assert again == json.loads(json.dumps(res))
+    table = json.loads((tmp_path / "t.json").read_text())
+    assert list(table) == [T.cache_key(net, wl)]
+    # another connectome or batch is measured anew
+    net2, _ = fake(256, fp="beef")
+    assert T.tune(net2, wl, timer=lambda n, w, r, t: 2.0 if r == "push"
+                  else 3.0)["best"]["road"] == "push"
+
+
+def test_unwritable_cache_is_skipped(tmp_path, monkeypatch):
+    blocker = tmp_path / "file"
+    blocker.write_text("x")
+    monkeypatch.setenv("FLYBRAIN_TUNE_CACHE", str(blocker / "sub" / "t.json"))
+    net, wl = fake(32)
+    res = T.tune(net, wl, timer=lambda *a: 5.0)
+    assert res["best"] == {"road": "push", "pull_tile": None,
+                           "ms_per_window": 5.0}

@@ -11,6 +11,7 @@ import torch
 import oracle as O
 from flybrain_amd import connectome as C
 from flybrain_amd import sim as S
+from flybrain_amd.device import DeviceNet
 from gpu_helpers import tiny_conn
 
 pytestmark = pytest.mark.gpu
@@ -225,3 +226,27 @@ def test_roads_agree_after_restore_and_out_mask():
                                   engines[1].v.cpu().numpy().view(np.uint32))
     np.testing.assert_array_equal(
         engines[0].delay_buffer.numpy(), engines[1].delay_buffer.numpy())
+
+
+def test_tuned_runner_matches_default(tmp_path, monkeypatch):
+    """The autotuner times every (road, workgroup shape) candidate on the
+    workload itself and the runner built on its choice gives the same
+    counts as the default one: the shape changes speed, not results."""
+    from flybrain_amd import autotune, workloads as WL
+    from flybrain_amd.batch import BatchRunner
+
+    monkeypatch.setenv("FLYBRAIN_TUNE_CACHE", str(tmp_path / "tune.json"))
+    conn = C.synthetic(2048, None, mean_out=60, count_exp=1.2, n_groups=6,
+                       seed=31)
+    net = DeviceNet(conn)
+    wl = WL.trials_b64(conn.n, seed=3, batch=96, steps=400)
+    res = autotune.tune(net, wl, warmup_steps=40, windows=3)
+    assert len(res["times"]) == len(autotune.candidates(96))
+    assert all(ms > 0 for ms in res["times"].values())
+    assert (tmp_path / "tune.json").exists()
+    tuned = BatchRunner(net, wl, counts=True, road="tuned")
+    assert tuned.sim.road == res["best"]["road"]
+    base = BatchRunner(net, wl, counts=True)
+    tuned.advance(400, use_graph=False)
+    base.advance(400, use_graph=False)
+    np.testing.assert_array_equal(tuned.counts(), base.counts())

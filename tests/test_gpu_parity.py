@@ -149,6 +149,20 @@ def test_small10k_config1(golden):
 def test_philox_drive_batched(batch, road, fused):
     """Dense-rate Philox drive (Engine.step road) and per-trial drive tables
     against the oracle running the same Philox stream, trial by trial."""
+    philox_batched(batch, road, fused)
+
+
+@pytest.mark.parametrize("batch,tile", [(64, 128), (64, 512), (100, 128),
+                                        (256, 128), (256, 512), (600, 128),
+                                        (600, 512)])
+def test_pull_tiles_match_oracle(batch, tile):
+    """Every compiled pull-road workgroup shape (128 / 512 neuron-trials,
+    one to eight neurons per tile, or a part of one neuron's trials) gives
+    the oracle's raster and state."""
+    philox_batched(batch, "pull", False, tile)
+
+
+def philox_batched(batch, road, fused, tile=None):
     conn = C.synthetic(512, None, mean_out=40, count_exp=1.2, n_groups=4,
                        seed=21)
     n = conn.n
@@ -157,7 +171,8 @@ def test_philox_drive_batched(batch, road, fused):
     drv = [rng.choice(n, rng.integers(5, 40), replace=False) for _ in range(batch)]
     rate = [float(rng.uniform(50, 900)) for _ in range(batch)]
     net = DeviceNet(conn)
-    sim = DeviceSim(net, batch, seeds=seeds, raster_len=40, road=road)
+    sim = DeviceSim(net, batch, seeds=seeds, raster_len=40, road=road,
+                    pull_tile=tile)
     ne = np.concatenate(drv)
     tr = np.concatenate([np.full(len(d), b) for b, d in enumerate(drv)])
     sim.exempt_pairs(ne, tr)  # refractory exemption per trial
