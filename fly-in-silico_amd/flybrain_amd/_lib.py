@@ -35,6 +35,8 @@ def write_slot(u):
 def read_slot(t):
     """Ring slot that step ``t`` reads (the spikes of step t - DELAY)."""
     return (t + 1) % RING_LEN
+
+
 FIRE_OFF, OUT_OFF, IN_OFF, HOLD, NOREFRAC = 0x01, 0x02, 0x04, 0x08, 0x10
 BLOCK_WAVES, LINE_WORDS, LINE_INTS, EVENT_ALIGN = 4, 16, 32, 8
 REC_SLOTS, PULL_TILE, PULL_MAX = 40, 256, 1024
@@ -165,7 +167,7 @@ def lib():
             2: FisSim.w_scale.offset, 3: FisSim.refrac_steps.offset,
             4: FisSim.raster_len.offset, 5: EVENT_BYTES,
             6: ctypes.sizeof(FisDriveEntry), 7: FisSim.in_offsets.offset,
-            8: FisSim.t_origin.offset}
+            8: FisSim.t_origin.offset, 9: FisSim.pull_tile.offset}
     for k, v in want.items():
         got = L.fis_struct_layout(k)
         if got != v:

@@ -250,19 +250,21 @@ int fis_seed_ring(const fis_sim* s, int slot, void* stream);
  * the end of that step with the old flag.  For each listed neuron-trial
  * whose last step spiked (st bit 7), its row is added into ring slot `slot`
  * (flag now clear) or subtracted (flag now set).  fl must already hold the
- * new flags; `slot` is the slot that step's scatter wrote, clock mod 20. */
+ * new flags; `slot` is the slot that step's spikes went to:
+ * (clock - 1 + 21) mod FIS_RING_LEN. */
 int fis_resend(const fis_sim* s, const int32_t* idx, int n_idx, int slot,
                void* stream);
 
 /* ---- pull road: one fused launch per window (bstride a power of two,
- * 64 .. 1024).  Each workgroup owns max(512, bstride) neuron-trials
- * (that many / bstride neurons x all trials).  It sums the input of its neurons for the W steps
+ * 64 .. 1024).  Each workgroup owns pull_tile neuron-trials: pull_tile /
+ * bstride neurons x all trials, or one neuron x pull_tile of its trials
+ * when bstride > pull_tile.  It sums the input of its neurons for the W steps
  * t0 .. t0+W-1 from the spike records of steps t0-20 .. t0+W-21 over their
  * in-edges into LDS (exact int32), then runs the W update steps of
  * Engine._core (sim.py:920-953) from LDS and writes the window's own
  * spike records; finally clock += W.  The int32 ring is not used, except
  * that the first `ring_steps` steps of the window also consume (and clear)
- * ring slot (t+1) mod 20, which holds input carried over from a restore. */
+ * ring slot (t+1) mod 40, which holds input carried over from a restore. */
 int fis_pull_window(const fis_sim* s, int W, int ring_steps, void* stream);
 
 /* Pull-road counterpart of fis_resend: after FIS_OUT_OFF changed on the
