@@ -333,12 +333,18 @@ __device__ __forceinline__ DriveId drive_id(const fis_sim& s, int idx,
   return d;
 }
 
+// `fixed`: the table has one segment, and `p` is this lane's probability
+// (its group's rate times prob_scale, what TableRate gives at every step).
 template <int WMAX>
 __device__ __forceinline__ uint32_t fused_kicks(const fis_sim& s,
-                                                const DriveId& d, int neuron,
+                                                const DriveId& d, bool fixed,
+                                                float p, int neuron,
                                                 int64_t t0, int W) {
   if (d.group < 0) return 0u;
   const int64_t t = t0 - d.origin;
+  if (fixed)
+    return window_bits<WMAX>(t, W, (uint32_t)neuron, d.trial_id, d.seed,
+                             [p]() { return p; });
   return window_bits<WMAX>(t, W, (uint32_t)neuron, d.trial_id, d.seed,
                            TableRate(s.drive_rate, s.drive_n_groups,
                                      s.drive_n_seg, s.drive_seg_steps,
@@ -453,6 +459,45 @@ __device__ __forceinline__ uint32_t run_window(const fis_sim& s, Cell& c,
   return fired;
 }
 
+// A full window of lif_step for a wave whose neuron-trials carry no flag and
+// no Poisson kick in the window: the fire mask is on, nothing is held, the
+// gate threshold is refrac_steps, and the stimulus term is the zero
+// w_scale * (0.0 * gain), whose sum with the finite v is v (value-equal; the
+// add is left out).  The refractory counter is kept as an offset: after the
+// update of step k it is k + b, so the gate r >= R is b >= R - k (a scalar
+// per step) and a spike at step k sets b = -(k + 1) for step k + 1.  Equal
+// in every value to run_window with those inputs; returns the steps at
+// which the neuron-trial fired (bit k = step k).
+template <int WMAX>
+__device__ __forceinline__ uint32_t run_window_plain(const fis_sim& s, Cell& c,
+                                                     const int32_t (&in)[WMAX]) {
+  int b = c.sp ? 0 : c.r + 1;  // counter after the update of step 0
+  float v = c.v, g = c.g;
+  bool fire = false;
+  uint32_t fired = 0;
+#pragma unroll
+  for (int k = 0; k < WMAX; ++k) {
+    const float rec = s.w_scale * (float)in[k];
+    const float gd = g * s.a_syn;
+    const float g_new = (b >= s.refrac_steps - k) ? gd + rec : gd;
+    const float t1 = v - s.v_rest;
+    const float t2 = g - t1;
+    const float t3 = s.a_mem * t2;
+    const float vv = v + t3;
+    fire = vv > s.v_th;
+    const float t4 = vv - s.v_reset;
+    v = fire ? vv - t4 : vv;
+    g = fire ? 0.0f : g_new;
+    fired |= fire ? (1u << k) : 0u;
+    if (k + 1 < WMAX) b = fire ? -(k + 1) : b;
+  }
+  c.v = v;
+  c.g = g;
+  c.sp = fire;
+  c.r = (WMAX - 1) + b;  // the counter after the last update
+  return fired;
+}
+
 // One event's CSR row, read once and added into every trial of its mask.
 __device__ __forceinline__ void scatter_event(const fis_sim& s,
                                               const fis_event& ev, int lane) {
@@ -513,6 +558,14 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
   const int pre = idx / bs;       // neuron of this lane
   const int tb = idx - pre * bs;  // trial of this lane
   const DriveId drive = act ? drive_id(s, idx, tb) : DriveId{-1, 0u, 0ull, 0};
+  // A drive table of one segment and at most 64 rates is read by every
+  // wave, one rate per lane, alongside the lanes' groups: a driven lane's
+  // rate is then a lane shuffle rather than a second dependent load, and
+  // no segment is computed (uniform over the grid).
+  const bool lane_table = s.drive_group && s.drive_n_seg == 1 &&
+                          s.drive_n_groups <= kWave;
+  float lane_rate = 0.0f;
+  if (lane_table && lane < s.drive_n_groups) lane_rate = s.drive_rate[lane];
 
   // the window's ring slots, all read (and the non-zero ones cleared)
   // before the recurrence starts: nothing in it waits for memory, and the
@@ -531,7 +584,11 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
   float v = 0.0f, g = 0.0f, vh = 0.0f;
   uint32_t st = 0, fl = FIS_FIRE_OFF, sb = 0;
   if (act) load_cell(s, idx, v, g, st, fl, sb, vh);
-  sb |= fused_kicks<WMAX>(s, drive, pre, t0, W);
+  float p_fixed = 0.0f;
+  if (lane_table)  // every lane takes part in the shuffle
+    p_fixed = __shfl(lane_rate, drive.group < 0 ? 0 : drive.group) *
+              s.prob_scale;
+  sb |= fused_kicks<WMAX>(s, drive, lane_table, p_fixed, pre, t0, W);
 
   Cell c = make_cell(s, fl, vh, st, v, g);
   const Stim stim = make_stim(s);
@@ -1006,9 +1063,13 @@ void k_pull(fis_sim s, int W,
     }
   }
 
-  // bit k: this lane fired at step k
+  // bit k: this lane fired at step k.  Waves with no flag, no kick and no
+  // trial end in a full window (most of them) take the plain recurrence.
+  const bool plain =
+      W == WMAX && !any_lane(!act || fl != 0u || sb != 0u || kstop != W);
   const uint32_t fired_bits =
-      act ? run_window<WMAX>(s, c, in, sb, stim, W) : 0u;
+      plain ? run_window_plain<WMAX>(s, c, in)
+            : (act ? run_window<WMAX>(s, c, in, sb, stim, W) : 0u);
   const uint32_t nspk = __popc(fired_bits & ((1u << kstop) - 1u));
   if (act) store_cell(s, c, idx, pre, nspk);
 

@@ -30,9 +30,10 @@ MAX_REFRAC_STEPS = 126
 
 
 def derived(params=None) -> dict:
-    p = dict(MODEL_PARAMS)
-    if params:
-        p.update(params)
+    """Per-step constants of a full parameter dict (``params`` replaces
+    MODEL_PARAMS whole, as ``Engine.__init__`` takes it, sim.py:350: a key
+    it lacks raises KeyError)."""
+    p = dict(params or MODEL_PARAMS)
     f = np.float32
     refrac_steps = int(round(p["tRefrac"] / DT))
     if not 0 <= refrac_steps <= MAX_REFRAC_STEPS:

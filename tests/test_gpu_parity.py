@@ -162,7 +162,20 @@ def test_pull_tiles_match_oracle(batch, tile):
     philox_batched(batch, "pull", False, tile)
 
 
-def philox_batched(batch, road, fused, tile=None):
+@pytest.mark.parametrize("batch,road,refrac", [(64, "pull", 0),
+                                                (64, "pull", 3),
+                                                (256, "pull", 7),
+                                                (64, "pull", 126),
+                                                (3, "push", 3)])
+def test_refractory_periods_match_oracle(batch, road, refrac):
+    """Refractory periods shorter than a window (the gate reopens within
+    it), zero, and the longest the engine takes: raster and state equal the
+    oracle, on the pull road's plain and flagged waves alike."""
+    philox_batched(batch, road, False,
+                   params=dict(O.MODEL_PARAMS, tRefrac=refrac * 0.1))
+
+
+def philox_batched(batch, road, fused, tile=None, params=None):
     conn = C.synthetic(512, None, mean_out=40, count_exp=1.2, n_groups=4,
                        seed=21)
     n = conn.n
@@ -171,15 +184,18 @@ def philox_batched(batch, road, fused, tile=None):
     drv = [rng.choice(n, rng.integers(5, 40), replace=False) for _ in range(batch)]
     rate = [float(rng.uniform(50, 900)) for _ in range(batch)]
     net = DeviceNet(conn)
-    sim = DeviceSim(net, batch, seeds=seeds, raster_len=40, road=road,
-                    pull_tile=tile)
+    sim = DeviceSim(net, batch, params=params, seeds=seeds, raster_len=40,
+                    road=road, pull_tile=tile)
+    if params:
+        assert sim.c["refrac_steps"] == round(params["tRefrac"] / 0.1)
     ne = np.concatenate(drv)
     tr = np.concatenate([np.full(len(d), b) for b, d in enumerate(drv)])
     sim.exempt_pairs(ne, tr)  # refractory exemption per trial
     table = DriveTable(sim, ne, tr, tr, np.array(rate, np.float32))
     if fused:  # the update launches draw the table's kicks themselves
         sim.fuse_drive(table)
-    ora = O.OracleEngine(conn, batch, seeds=seeds, no_refractory_trials=drv)
+    ora = O.OracleEngine(conn, batch, params=params, seeds=seeds,
+                         no_refractory_trials=drv)
     rates = np.zeros((batch, n), np.float32)
     for b, d in enumerate(drv):
         rates[b, d] = rate[b]
