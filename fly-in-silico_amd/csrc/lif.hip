@@ -130,6 +130,12 @@ struct TableRate {
 };
 
 // trial b's own step count at global step t (0 at its slot's start)
+// The current step: the clock, plus the steps a deferred run of windows
+// has launched since the clock was last advanced.
+__device__ __forceinline__ int64_t now(const fis_sim& s) {
+  return *s.clock + s.clock_ahead;
+}
+
 __device__ __forceinline__ int64_t origin_of(const fis_sim& s, int b) {
   return s.t_origin ? s.t_origin[b] : 0;
 }
@@ -157,7 +163,7 @@ __global__ __launch_bounds__(kBlock) void k_drive_rates(fis_sim s,
   if (b < s.batch) {
     const float p = rates[(int64_t)b * s.n + i] * s.prob_scale;
     if (p > 0.0f)
-      bits = window_bits<FIS_MAX_WINDOW>(*s.clock - origin_of(s, b), W,
+      bits = window_bits<FIS_MAX_WINDOW>(now(s) - origin_of(s, b), W,
                                          (uint32_t)i, s.trial_ids[b],
                                          s.seeds[b], [p]() { return p; });
   }
@@ -172,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_drive_table(
   const fis_drive_entry e = entries[j];
   const int i = (int)e.idx / s.bstride;
   const int b = (int)e.idx - i * s.bstride;
-  const int64_t t = *s.clock - origin_of(s, b);
+  const int64_t t = now(s) - origin_of(s, b);
   const uint32_t bits = window_bits<FIS_MAX_WINDOW>(
       t, W, (uint32_t)i, s.trial_ids[b], s.seeds[b],
       TableRate(rate_hz, n_groups, n_seg, seg_steps, e.group, s.prob_scale,
@@ -187,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void k_drive_tape(fis_sim s,
                                                        const int32_t* tape_idx,
                                                        int64_t n_tape_steps) {
   // blockIdx.y = window step k
-  const int64_t t = *s.clock + blockIdx.y;
+  const int64_t t = now(s) + blockIdx.y;
   if (t >= n_tape_steps) return;
   const int64_t beg = tape_ptr[t], end = tape_ptr[t + 1];
   for (int64_t e = beg + (int64_t)blockIdx.x * kBlock + threadIdx.x; e < end;
@@ -554,7 +560,7 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
   const int wave = idx >> 6;
   const int nblk = (int)gridDim.x;
   const bool act = idx < nb;
-  const int64_t t0 = *s.clock;
+  const int64_t t0 = now(s);
   const int pre = idx / bs;       // neuron of this lane
   const int tb = idx - pre * bs;  // trial of this lane
   const DriveId drive = act ? drive_id(s, idx, tb) : DriveId{-1, 0u, 0ull, 0};
@@ -673,7 +679,7 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
 __global__ __launch_bounds__(kBlock) void k_scatter(fis_sim s, int W) {
   // the window is complete: the next drive and update start W steps later
   // (no kernel of this launch reads the clock)
-  if (blockIdx.x == 0 && threadIdx.x == 0) *s.clock += W;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && !s.defer_tick) *s.clock += W;
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t nw = n_waves_of(s);
   const int64_t stride = (int64_t)gridDim.x * (kBlock / kWave);
@@ -692,7 +698,8 @@ __global__ __launch_bounds__(kBlock) void k_scatter(fis_sim s, int W) {
 // into one or two ring slots (142 MB each at 138,639 x 256), a working set
 // the 256 MB memory-side cache can hold.
 __global__ __launch_bounds__(kBlock) void k_scatter_steps(fis_sim s, int W) {
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *s.clock += W;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && !s.defer_tick)
+    *s.clock += W;
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t nw = n_waves_of(s);
   const int k = blockIdx.y;
@@ -895,7 +902,7 @@ void k_pull(fis_sim s, int W,
   const int part = tile % parts;       // which part of its neuron(s)
   const int w0 = part * wpt;           // first word of the tile
   const int np = min(P, s.n - p0);
-  const int64_t t0 = *s.clock;
+  const int64_t t0 = now(s);
   const int rbase = (int)((t0 + kRec - 20) % kRec);  // slot of step t0-20
   const int wbase = (int)(t0 % kRec);                // slot of step t0
   const int pstride = kRec * words;                  // words per neuron
@@ -1129,8 +1136,8 @@ void k_pull(fis_sim s, int W,
   }
 }
 
-// clock += W after a pull window (one thread; every workgroup of the window
-// has read the clock by then)
+// clock += W after a window (one thread; every workgroup of the window has
+// read the clock by then), or after a deferred run of windows
 __global__ void k_tick(int64_t* clock, int W) { *clock += W; }
 
 // Re-mask the last step's record of each listed neuron-trial with its
@@ -1147,7 +1154,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_resend(fis_sim s,
   const int pre = src / s.bstride;
   const int b = src - pre * s.bstride;
   const int words = s.bstride >> 6;
-  const int slot = (int)((*s.clock + kRec - 1) % kRec);
+  const int slot = (int)((now(s) + kRec - 1) % kRec);
   uint64_t* w = s.rec + ((int64_t)pre * kRec + slot) * words + (b >> 6);
   const uint64_t bit = 1ull << (b & 63);
   if (s.fl[src] & FIS_OUT_OFF) {
@@ -1170,7 +1177,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_to_ring(fis_sim s,
   const int lane = threadIdx.x & (kWave - 1);
   const int pre = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6);
   if (pre >= s.n) return;
-  const int64_t u = *s.clock - 20 + (int64_t)blockIdx.y;
+  const int64_t u = now(s) - 20 + (int64_t)blockIdx.y;
   if (u < 0) return;
   const int slot = (int)(u % kRec);
   const int words = s.bstride >> 6;
@@ -1348,6 +1355,7 @@ int64_t fis_struct_layout(int what) {
     case 7: return (int64_t)offsetof(fis_sim, in_offsets);
     case 8: return (int64_t)offsetof(fis_sim, t_origin);
     case 9: return (int64_t)offsetof(fis_sim, pull_tile);
+    case 10: return (int64_t)offsetof(fis_sim, clock_ahead);
     default: return -1;
   }
 }
@@ -1433,6 +1441,13 @@ int fis_update(const fis_sim* s, int W, void* stream) {
                        dim3(kBlock), 0, (hipStream_t)stream, *s, W);
   return (int)hipGetLastError();
 }
+int fis_tick(const fis_sim* s, int steps, void* stream) {
+  if (!s || !s->clock || steps < 0) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, (hipStream_t)stream,
+                     s->clock, steps);
+  return (int)hipGetLastError();
+}
+
 int fis_update_scatter(const fis_sim* s, int W, void* stream) {
   if (int rc = check_sim(s)) return rc;
   if (W < 1 || W > FIS_MAX_WINDOW || s->bstride >= kWave)
@@ -1444,8 +1459,9 @@ int fis_update_scatter(const fis_sim* s, int W, void* stream) {
   else
     hipLaunchKernelGGL((k_update<FIS_MAX_WINDOW, true>), dim3(grid_for(nb)),
                        dim3(kBlock), 0, (hipStream_t)stream, *s, W);
-  hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, (hipStream_t)stream,
-                     s->clock, W);
+  if (!s->defer_tick)
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, (hipStream_t)stream,
+                       s->clock, W);
   return (int)hipGetLastError();
 }
 
@@ -1516,8 +1532,9 @@ int fis_pull_window(const fis_sim* s, int W, int ring_steps, void* stream) {
   const int per_xcd = ((n_tiles + 7) / 8 + parts - 1) / parts * parts;
   hipLaunchKernelGGL(kern, dim3((unsigned)(8 * per_xcd)), dim3(tile), 0,
                      (hipStream_t)stream, *s, W, ring_steps, n_tiles, per_xcd);
-  hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, (hipStream_t)stream,
-                     s->clock, W);
+  if (!s->defer_tick)
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, (hipStream_t)stream,
+                       s->clock, W);
   return (int)hipGetLastError();
 }
 

@@ -20,7 +20,7 @@ CSRC = PKG.parent / "csrc"
 SOURCES = [CSRC / "lif.hip"]
 HEADER = ROOT / "include" / "fis.h"
 LIB_PATH = PKG / "libfis.so"
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 MAX_WINDOW = 20
 RING_LEN = 40  # int32 ring slots (two delays)
@@ -103,6 +103,7 @@ class FisSim(ctypes.Structure):
         ("drive_n_groups", ctypes.c_int32), ("drive_n_seg", ctypes.c_int32),
         ("drive_seg_steps", ctypes.c_int32),
         ("pull_tile", ctypes.c_int32),
+        ("clock_ahead", ctypes.c_int32), ("defer_tick", ctypes.c_int32),
     ]
 
 
@@ -145,6 +146,7 @@ def lib():
         "fis_update": ([sims, I, P], I),
         "fis_scatter": ([sims, I, P], I),
         "fis_update_scatter": ([sims, I, P], I),
+        "fis_tick": ([sims, I, P], I),
         "fis_seed_ring": ([sims, I, P], I),
         "fis_resend": ([sims, P, I, I, P], I),
         "fis_pull_window": ([sims, I, I, P], I),
@@ -167,7 +169,8 @@ def lib():
             2: FisSim.w_scale.offset, 3: FisSim.refrac_steps.offset,
             4: FisSim.raster_len.offset, 5: EVENT_BYTES,
             6: ctypes.sizeof(FisDriveEntry), 7: FisSim.in_offsets.offset,
-            8: FisSim.t_origin.offset, 9: FisSim.pull_tile.offset}
+            8: FisSim.t_origin.offset, 9: FisSim.pull_tile.offset,
+            10: FisSim.clock_ahead.offset}
     for k, v in want.items():
         got = L.fis_struct_layout(k)
         if got != v:

@@ -93,7 +93,8 @@ class BatchRunner:
         torch.cuda.current_stream().wait_stream(side)
         g = torch.cuda.CUDAGraph()
         try:
-            with torch.cuda.graph(g):
+            # one clock launch per replay, not one per window
+            with torch.cuda.graph(g), s.deferred_clock():
                 for _ in range(graph_windows):
                     self._window(self.window)
         finally:

@@ -14,6 +14,7 @@ ring 40 x 4 B -- about 174 B; at 138,639 neurons x 256 trials that is
 
 from __future__ import annotations
 
+import contextlib
 import ctypes
 
 import numpy as np
@@ -268,6 +269,9 @@ class DeviceSim:
         dims = getattr(self, "_drive_dims", (0, 0, 0))
         s.drive_n_groups, s.drive_n_seg, s.drive_seg_steps = dims
         s.pull_tile = self.pull_tile
+        base = getattr(self, "_defer_base", None)
+        if base is not None:  # inside deferred_clock
+            s.clock_ahead, s.defer_tick = self.t - base, 1
         self.struct = s
         self._sp = ctypes.byref(s)
         self.version = getattr(self, "version", 0) + 1
@@ -506,7 +510,7 @@ class DeviceSim:
         self._push_only("scatter")
         W = self._open_window
         L.check(self.lib.fis_scatter(self._sp, W, _stream()), "fis_scatter")
-        self.t += W
+        self._moved(W)
 
     def _push_only(self, what):
         if self.road != "push":
@@ -518,7 +522,7 @@ class DeviceSim:
         ring_steps = max(0, min(W, self.ring_until - self.t))
         L.check(self.lib.fis_pull_window(self._sp, W, ring_steps, _stream()),
                 "fis_pull_window")
-        self.t += W
+        self._moved(W)
 
     def step_launch(self, W):
         """The road's launches of one window after its drive."""
@@ -536,7 +540,37 @@ class DeviceSim:
         self._push_only("update_scatter")
         L.check(self.lib.fis_update_scatter(self._sp, W, _stream()),
                 "fis_update_scatter")
+        self._moved(W)
+
+    def _moved(self, W):
+        """The host clock moves on by a window of W steps.  In a deferred
+        run the device clock stays put, and the next window's kernels read
+        it plus the steps launched since the run began."""
         self.t += W
+        base = getattr(self, "_defer_base", None)
+        if base is not None:
+            self.struct.clock_ahead = self.t - base
+
+    @contextlib.contextmanager
+    def deferred_clock(self):
+        """Windows launched inside leave the device clock alone (each reads
+        it plus the steps launched before it in the run) and one
+        ``fis_tick`` at the end advances it by the whole run: for a graph
+        of several windows, one clock launch per replay instead of one per
+        window (at B = 1 that launch is a quarter of the window)."""
+        if getattr(self, "_defer_base", None) is not None:
+            raise RuntimeError("deferred_clock does not nest")
+        self._defer_base = self.t
+        self.struct.clock_ahead, self.struct.defer_tick = 0, 1
+        try:
+            yield self
+        finally:
+            steps = self.t - self._defer_base
+            self._defer_base = None
+            self.struct.clock_ahead, self.struct.defer_tick = 0, 0
+            if steps:
+                L.check(self.lib.fis_tick(self._sp, steps, _stream()),
+                        "fis_tick")
 
     def window(self, W, drive=None):
         """One window: drive (a DriveTable, Tape or rates tensor), update,

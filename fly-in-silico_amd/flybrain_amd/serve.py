@@ -26,6 +26,7 @@ the same request run alone; ``tests/test_gpu_serve.py`` checks this.
 
 from __future__ import annotations
 
+import contextlib
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -238,7 +239,10 @@ class Server:
         g = torch.cuda.CUDAGraph()
         t0 = [sh.sim.t for sh in self.shards]  # capture only records
         try:
-            with torch.cuda.graph(g):
+            with torch.cuda.graph(g), contextlib.ExitStack() as clocks:
+                # one clock launch per shard and replay, not per window
+                for sh in self.shards:
+                    clocks.enter_context(sh.sim.deferred_clock())
                 for _ in range(self.graph_windows):
                     for sh in self.shards:
                         sh.window(self.window)

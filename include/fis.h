@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define FIS_ABI_VERSION 8
+#define FIS_ABI_VERSION 9
 #define FIS_MAX_WINDOW 20   /* steps per window: delay horizon 19 + 1     */
 #define FIS_RING_LEN 40     /* int32 ring slots, two delays: the spikes of
                                step u are summed into slot (u + 21) mod 40,
@@ -106,7 +106,8 @@ typedef struct {
   fis_event* events;      /* [n_waves * ev_cap]                        */
   int32_t* ev_count;      /* [n_blocks][32]: count of wave w at
                              [w / 4][w % 4]                            */
-  int64_t* clock;         /* [1] first step of the next window         */
+  int64_t* clock;         /* [1] first step of the next window (plus
+                             clock_ahead, below)                       */
   /* per-trial Philox key and stream id, [bstride] */
   const uint64_t* seeds;
   const uint32_t* trial_ids;
@@ -177,6 +178,14 @@ typedef struct {
    * layout of `act` ([n][max(1, bstride / pull_tile)] words).  Chosen by
    * measurement (flybrain_amd.autotune). */
   int32_t pull_tile;
+  /* Deferred clock, for a run of windows captured into one graph.  Every
+   * kernel takes *clock + clock_ahead as the current step; with
+   * defer_tick nonzero the window entry points leave *clock alone, and
+   * the caller advances it once with fis_tick after the run (a
+   * one-thread launch per window cost ~3 us of a 12 us window at B = 1).
+   * Both 0 outside such a run. */
+  int32_t clock_ahead;
+  int32_t defer_tick;
 } fis_sim;
 
 /* Drive table entry: neuron-trial `idx` is driven at rate
@@ -230,14 +239,19 @@ int fis_update(const fis_sim* s, int W, void* stream);
 
 /* Scatter of the events of the window just updated (W steps) into the
  * int32 ring (the SpMV of sim.py:906-918 / ev_spmv.cu:60-79, integer,
- * exact), then clock += W. */
+ * exact), then clock += W (unless defer_tick). */
 int fis_scatter(const fis_sim* s, int W, void* stream);
 
 /* fis_update + fis_scatter without the event records, bstride < 64 only:
  * each wave adds its window's spikes straight into the ring slots that no
  * step of the window reads (Engine._core + the SpMV, sim.py:906-953); a
- * one-thread launch then advances the clock by W. */
+ * one-thread launch then advances the clock by W (unless defer_tick). */
 int fis_update_scatter(const fis_sim* s, int W, void* stream);
+
+/* *clock += steps (one thread): closes a run of windows launched with
+ * defer_tick set.  No counterpart in the reference, whose step counter is
+ * a host integer (sim.py:745-771). */
+int fis_tick(const fis_sim* s, int steps, void* stream);
 
 /* Scatter the current spike state (st bit 7, not FIS_OUT_OFF) into ring
  * slot `slot` -- used after a restore to rebuild the input in flight. */
@@ -262,7 +276,8 @@ int fis_resend(const fis_sim* s, const int32_t* idx, int n_idx, int slot,
  * t0 .. t0+W-1 from the spike records of steps t0-20 .. t0+W-21 over their
  * in-edges into LDS (exact int32), then runs the W update steps of
  * Engine._core (sim.py:920-953) from LDS and writes the window's own
- * spike records; finally clock += W.  The int32 ring is not used, except
+ * spike records; finally clock += W (unless defer_tick).  The int32 ring
+ * is not used, except
  * that the first `ring_steps` steps of the window also consume (and clear)
  * ring slot (t+1) mod 40, which holds input carried over from a restore. */
 int fis_pull_window(const fis_sim* s, int W, int ring_steps, void* stream);

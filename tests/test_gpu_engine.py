@@ -250,3 +250,50 @@ def test_tuned_runner_matches_default(tmp_path, monkeypatch):
     tuned.advance(400, use_graph=False)
     base.advance(400, use_graph=False)
     np.testing.assert_array_equal(tuned.counts(), base.counts())
+
+
+@pytest.mark.parametrize("batch,road,fused", [(1, "push", True),
+                                              (3, "push", False),
+                                              (64, "push", False),
+                                              (64, "pull", False),
+                                              (256, "pull", False)])
+def test_graph_replays_equal_eager_windows(batch, road, fused):
+    """A captured graph of several windows leaves the device clock alone
+    until its one closing tick (each window reads the clock plus the steps
+    launched before it).  Replay after replay, with eager windows between,
+    it gives the counts, state and clock of the same windows run one by
+    one; the drive table's rates change every 7 steps inside a replay."""
+    from flybrain_amd import workloads as WL
+    from flybrain_amd.batch import BatchRunner
+
+    conn = C.synthetic(1024, None, mean_out=50, count_exp=1.2, n_groups=4,
+                       seed=37)
+    net = DeviceNet(conn)
+    if batch == 1:
+        wl = WL.plain_b1(conn.n, seed=5, steps=1000, n_driven=60,
+                         rate_hz=400.0)
+    else:
+        wl = WL.trials_b64(conn.n, seed=5, batch=batch, steps=1000)
+    if not fused:  # rates that change every 7 steps, also inside windows
+        n_groups = wl.rate_hz.shape[1]
+        rng = np.random.default_rng(batch)
+        wl.rate_hz = rng.uniform(0.0, 600.0, size=(150, n_groups)).astype(
+            np.float32)
+        wl.seg_steps = 7
+    graph = BatchRunner(net, wl, counts=True, road=road, fused_drive=fused)
+    eager = BatchRunner(net, wl, counts=True, road=road, fused_drive=fused)
+    assert graph.fused == eager.fused == fused
+    graph.capture(3)
+    for rep in range(3):
+        graph.advance(60)  # one replay of three 20-step windows
+        eager.advance(60, use_graph=False)
+        torch.cuda.synchronize()
+        assert graph.t == eager.t
+        assert int(graph.sim.clock.item()) == int(eager.sim.clock.item()) \
+            == eager.t, "clock after replay %d" % rep
+        np.testing.assert_array_equal(graph.counts(), eager.counts(),
+                                      err_msg="counts after replay %d" % rep)
+        torch.testing.assert_close(graph.sim.v, eager.sim.v, rtol=0, atol=0)
+        graph.advance(20, use_graph=False)  # an eager window between
+        eager.advance(20, use_graph=False)
+    assert eager.counts().sum() > 0
