@@ -851,9 +851,10 @@ __device__ __forceinline__ Pair deal_pair(int pi, int excl, uint32_t m,
 }
 
 // Waves of a one-neuron tile that read the same in-edges and share their
-// rounds of pairs.  Measured at 256 trials, window ms: 1 (each wave its own
-// in-edges) 1.136; 4 (all waves) 0.98; 2 0.918 -- pairs of waves balance the
-// typical 100-odd in-edges and still give each wave two rounds to overlap.
+// rounds of pairs.  Measured at 256 trials, window ms (drive + pull): 1
+// (each wave its own in-edges) 1.014; 4 (all waves) 0.975; 2 0.898 --
+// pairs of waves balance the typical 100-odd in-edges and still give each
+// wave two rounds to overlap.
 constexpr int kPullShare = 2;
 
 // Neuron-trials per workgroup (fis_sim.pull_tile; 0 = FIS_PULL_TILE).  256
