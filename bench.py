@@ -6,10 +6,14 @@ Workloads (138,639 neurons, 15,091,983 edges, synthetic, seeded):
             m_out masks, per-trial Philox seeds; windows of 20 steps replayed
             from one captured HIP graph
   b1        plain_b1: B = 1, 20 neurons at 150 Hz -- the plain engine
+  b64       trials_b64: B = 64 trials, each driving its own 20-200 neurons
+            at U(50, 250) Hz with its own Philox seed; road and pull-road
+            workgroup shape (trial-block width) autotuned
 
 A bench "step" is one 0.1 ms model step of every trial.  ``value`` is
 B * steps * 1e-4 / wall seconds of the timed region (inputs and state
-resident in HBM).  The JSON line also carries the B = 1 rate, their ratio,
+resident in HBM).  The JSON line also carries the B = 1 and B = 64 rates,
+the B = 256 : B = 1 ratio,
 the roofline of the dominant kernel (per-launch durations from HIP events
 on the launch stream) and a CPU baseline: the oracle's torch restatement of
 the reference step (csrmv via torch.sparse.mm) on this host's cores, over a
@@ -250,10 +254,12 @@ def main():
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--b1-steps", type=int, default=10_000)
+    ap.add_argument("--b64-steps", type=int, default=10_000)
     ap.add_argument("--graph-windows", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=("all", "b256", "b1"), default="all")
+    ap.add_argument("--only", choices=("all", "b256", "b1", "b64"),
+                    default="all")
     ap.add_argument("--serve-requests", type=int, default=128,
                     help="requests of the serving run (0: skip it)")
     ap.add_argument("--serve-rate", type=float, default=200.0,
@@ -301,10 +307,15 @@ def main():
            "data": "synthetic"}
 
     res = {}
-    for name in (("b256", "b1") if args.only == "all" else (args.only,)):
+    for name in (("b256", "b1", "b64") if args.only == "all"
+                 else (args.only,)):
         if name == "b256":
             wl = WL.sensory_b256(conn.n, seed=0, steps=args.warmup + args.steps)
             steps = args.steps
+        elif name == "b64":  # per-trial drive, autotuned trial-block width
+            wl = WL.trials_b64(conn.n, seed=0,
+                               steps=args.warmup + args.b64_steps)
+            steps = args.b64_steps
         else:
             wl = WL.plain_b1(conn.n, seed=0, steps=args.warmup + args.b1_steps)
             steps = args.b1_steps
@@ -341,7 +352,7 @@ def main():
         log("calibration copy: 256 MiB read + 256 MiB written (%s)"
             % float(b[0]))
         del a, b
-    head = res.get("b256") or res.get("b1")
+    head = res.get("b256") or res.get("b1") or res.get("b64")
     wl = head["wl"]
     n, B, W = conn.n, wl.batch, head["runner"].window
     kt = head["kt"]
@@ -399,6 +410,20 @@ def main():
                                     for k, v in res["b1"]["kt"].items()}}
         out["ratio_b256_over_b1"] = round(res["b256"]["tps"]
                                           / res["b1"]["tps"], 3)
+    if "b64" in res:  # 64 trials, each with its own 20-200 driven neurons
+        r64 = res["b64"]
+        out["b64"] = {"workload": r64["wl"].name,
+                      "value": round(r64["tps"], 3), "steps": r64["steps"],
+                      "ms_per_step": round(1e3 * r64["wall"]
+                                           / r64["steps"], 6),
+                      "road": r64["runner"].sim.road,
+                      "pull_tile": (r64["runner"].sim.pull_tile
+                                    if r64["runner"].sim.road == "pull"
+                                    else None),
+                      "autotuned": (r64["tuned"]["times"] if r64["tuned"]
+                                    else None),
+                      "kernels_ms": {k: round(v, 5)
+                                     for k, v in r64["kt"].items()}}
     for name, rr in res.items():
         out.setdefault("spikes_per_trial_step", {})[name] = round(
             rr["spikes"] / (rr["wl"].batch * (rr["steps"] + args.warmup)),
