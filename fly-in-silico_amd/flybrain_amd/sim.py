@@ -33,7 +33,7 @@ import torch
 from . import _lib as L
 from . import connectome as C
 from .device import (MAX_RATE_HZ, DeviceNet, DeviceSim, DriveTable,  # noqa: F401
-                     RateTooHigh, Tape, check_rates_hz)
+                     RateTooHigh, Tape, check_rates_hz, pick_road)
 from .params import DT, MODEL_PARAMS
 
 KernelUnavailable = L.KernelUnavailable
@@ -187,9 +187,7 @@ class Engine:
     def __init__(self, weights, n, params=None, device="cuda", batch=1,
                  seed=0, no_refractory=None, spmv=None, allow_inexact=False,
                  road="auto"):
-        p = dict(MODEL_PARAMS)
-        if params:
-            p.update(params)
+        p = dict(params or MODEL_PARAMS)  # taken whole, as sim.py:350
         self.p = p
         self.n = int(n)
         self.batch = int(batch)
@@ -233,11 +231,17 @@ class Engine:
                     else np.asarray(no_refractory, dtype=np.int64))
         self._free = free
         # road: "push" (event scatter into the int32 ring) or "pull"
-        # (per-window gather from spike records, 64..1024 trials); "auto"
-        # takes the pull road from 64 trials up.  Same results either way.
+        # (per-window gather from spike records, 64..1024 trials).  step()
+        # advances one step per call, and the pull road's gather costs
+        # nearly as much for one step as for a window of 20 (measured per
+        # step at 138,639 neurons: B = 64 214 us pull against 124 us push,
+        # B = 256 685 against 530), so "auto" is the push road here; run()
+        # advances in windows of up to 20 and takes the pull road from 64
+        # trials up.  Same results either way.
         self._sim = DeviceSim(self._net, self.batch, params=p,
                               seeds=[int(seed) & ((1 << 64) - 1)] * self.batch,
-                              no_refractory=free, spikes_out=True, road=road)
+                              no_refractory=free, spikes_out=True,
+                              road="push" if road == "auto" else road)
         self.road = self._sim.road
         self.gen = _Gen(self, seed)
         self.refrac_steps = torch.full((self.n,), float(round(p["tRefrac"] / DT)),
@@ -695,7 +699,7 @@ def run(stimuli, silenced=None, config: RunConfig | None = None, neurons=None,
                           params=p, device=device, batch=B, seed=config.seed,
                           no_refractory=stim_all, graphs=False,
                           spmv=getattr(config, "spmv", None),
-                          road=getattr(config, "road", "auto"))
+                          road=pick_road(B, getattr(config, "road", "auto")))
     sim = engine._sim
     sim.pop_counts = torch.zeros(n, dtype=torch.int32, device=sim.device)
     sim._sync_struct()
