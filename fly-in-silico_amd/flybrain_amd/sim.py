@@ -243,6 +243,7 @@ class Engine:
                               no_refractory=free, spikes_out=True,
                               road="push" if road == "auto" else road)
         self.road = self._sim.road
+        self._clock_run = None  # an open deferred-clock run of eager steps
         self.gen = _Gen(self, seed)
         self.refrac_steps = torch.full((self.n,), float(round(p["tRefrac"] / DT)),
                                        dtype=torch.float32, device=dev)
@@ -375,12 +376,31 @@ class Engine:
         self._set_clock(self._sim.t + 1)
 
     def _set_clock(self, t):
+        self._clock_flush()
         s = self._sim
         s.t = int(t)
         s.clock.fill_(int(t))
 
+    # Eager steps run as one long deferred-clock run: each launch carries the
+    # steps taken since the device clock last moved (clock_ahead), and the
+    # clock catches up in one launch only when it is set, captured or a
+    # graph replays -- one launch fewer per step (B = 1: 19.0 -> 16.4 us).
+    def _clock_defer(self):
+        s = self._sim
+        if self._clock_run is not None and s.struct.clock_ahead >= (1 << 30):
+            self._clock_flush()  # keep clock_ahead far inside int32
+        if self._clock_run is None:
+            self._clock_run = s.deferred_clock()
+            self._clock_run.__enter__()
+
+    def _clock_flush(self):
+        run, self._clock_run = getattr(self, "_clock_run", None), None
+        if run is not None:
+            run.__exit__(None, None, None)
+
     # --- stepping -----------------------------------------------------------
     def _launch_step(self, rates):
+        self._clock_defer()
         s = self._sim
         s.drive_rates(rates, 1)
         s.step_launch(1)
@@ -400,17 +420,20 @@ class Engine:
             return False
         if self._graph is not None:
             return True
+        self._clock_flush()  # the graph's own step advances the clock
         snap = self._device_snapshot()
         try:
             side = torch.cuda.Stream()
             side.wait_stream(torch.cuda.current_stream())
             with torch.cuda.stream(side):
-                self._launch_step(self._rates)
+                self._sim.drive_rates(self._rates, 1)
+                self._sim.step_launch(1)
             torch.cuda.current_stream().wait_stream(side)
             self._device_restore(snap)
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
-                self._launch_step(self._rates)
+                self._sim.drive_rates(self._rates, 1)
+                self._sim.step_launch(1)
             self._device_restore(snap)
         except BaseException:
             self._device_restore(snap)
@@ -453,6 +476,7 @@ class Engine:
         if self._graph is not None and self._sim.t >= self._sim.ring_until:
             if r.data_ptr() != self._rates.data_ptr():
                 self._rates.copy_(r)
+            self._clock_flush()
             self._graph.replay()
             self._sim.t += 1
         else:
@@ -462,6 +486,7 @@ class Engine:
     def step_tape(self, tape: Tape):
         """One step with the kicks of a recorded tape (parity road)."""
         self.calls += 1
+        self._clock_defer()
         s = self._sim
         s.drive_tape(tape, 1)
         s.step_launch(1)
@@ -471,6 +496,7 @@ class Engine:
         """``steps`` steps in windows of up to 20 with one drive source
         (DriveTable, Tape or a [batch, n] rates tensor held constant)."""
         left = int(steps)
+        self._clock_defer()
         while left > 0:
             W = min(L.MAX_WINDOW, left)
             self._sim.window(W, drive)

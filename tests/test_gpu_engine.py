@@ -297,3 +297,36 @@ def test_graph_replays_equal_eager_windows(batch, road, fused):
         graph.advance(20, use_graph=False)  # an eager window between
         eager.advance(20, use_graph=False)
     assert eager.counts().sum() > 0
+
+
+@pytest.mark.parametrize("road", ["push", "pull"])
+def test_engine_switches_between_graph_and_direct_steps(road):
+    """Direct steps run as one deferred-clock run (each launch reads the
+    clock plus the steps since it last moved; the clock catches up when a
+    graph is captured or replayed): switching between direct and graph
+    steps several times keeps every step equal to the oracle's, and the
+    device clock ends equal to the step count."""
+    conn = C.synthetic(400, None, mean_out=40, count_exp=1.2, n_groups=3,
+                       seed=8)
+    n, B = conn.n, 3
+    drv = np.arange(0, n, 9)
+    rates = torch.zeros(B, n, device="cuda")
+    rates[:, drv] = 1500.0
+    rates[1, drv[::2]] = 3000.0
+    eng = S.Engine(conn, n, batch=B, seed=4321, no_refractory=drv, road=road)
+    ora = O.OracleEngine(conn, B, no_refractory=drv, seeds=[4321] * B)
+    r_np = rates.cpu().numpy()
+    t = 0
+    for graphs, steps in ((False, 45), (True, 30), (False, 25), (True, 20),
+                          (False, 13)):
+        assert eng.use_graphs(graphs) == graphs
+        for _ in range(steps):
+            sp = eng.step(rates)
+            ora.step_rates(r_np)
+            assert np.array_equal(sp.cpu().numpy(), ora.spikes), \
+                "graphs=%s, step %d" % (graphs, t)
+            t += 1
+    eng.use_graphs(True)  # closes the last direct run
+    torch.cuda.synchronize()
+    assert int(eng._sim.clock.item()) == t == eng._sim.t
+    assert ora.spikes.sum() > 0
