@@ -334,8 +334,8 @@ struct DriveId {
 __device__ __forceinline__ DriveId drive_id(const fis_sim& s, int idx,
                                             int trial) {
   DriveId d{-1, 0u, 0ull, 0};
-  if (s.drive_group) {
-    d.group = s.drive_group[idx];
+  if (s.drive_group || s.drive_dense) {
+    d.group = s.drive_group ? (int)s.drive_group[idx] : 0;
     d.trial_id = s.trial_ids[trial];
     d.seed = s.seeds[trial];
     d.origin = origin_of(s, trial);
@@ -352,9 +352,11 @@ __device__ __forceinline__ uint32_t fused_kicks(const fis_sim& s,
                                                 int64_t t0, int W) {
   if (d.group < 0) return 0u;
   const int64_t t = t0 - d.origin;
-  if (fixed)
+  if (fixed) {
+    if (!(p > 0.0f)) return 0u;  // no draw at all (k_drive_rates' test)
     return window_bits<WMAX>(t, W, (uint32_t)neuron, d.trial_id, d.seed,
                              [p]() { return p; });
+  }
   return window_bits<WMAX>(t, W, (uint32_t)neuron, d.trial_id, d.seed,
                            TableRate(s.drive_rate, s.drive_n_groups,
                                      s.drive_n_seg, s.drive_seg_steps,
@@ -598,7 +600,14 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
   if (lane_table)  // every lane takes part in the shuffle
     p_fixed = __shfl(lane_rate, drive.group < 0 ? 0 : drive.group) *
               s.prob_scale;
-  sb |= fused_kicks<WMAX>(s, drive, lane_table, p_fixed, pre, t0, W);
+  // dense rates: the lane's own probability, as k_drive_rates takes it
+  const bool dense = s.drive_dense != nullptr;
+  if (dense)
+    p_fixed = (act && tb < s.batch)
+                  ? s.drive_dense[(int64_t)tb * s.n + pre] * s.prob_scale
+                  : 0.0f;
+  sb |= fused_kicks<WMAX>(s, drive, lane_table || dense, p_fixed, pre, t0,
+                          W);
 
   Cell c = make_cell(s, fl, vh, st, v, g);
   const Stim stim = make_stim(s);
@@ -1312,6 +1321,7 @@ int check_sim(const fis_sim* s) {
   if (s->drive_group && (!s->drive_rate || s->drive_n_groups <= 0 ||
                          s->drive_n_seg <= 0 || s->drive_seg_steps <= 0))
     return (int)hipErrorInvalidValue;
+  if (s->drive_group && s->drive_dense) return (int)hipErrorInvalidValue;
   if ((int64_t)s->n * s->bstride >= ((int64_t)1 << 31) - kBlock)
     return (int)hipErrorInvalidValue;
   return 0;
@@ -1332,7 +1342,7 @@ int check_pull(const fis_sim* s) {
   // is bound by vector-instruction issue, and Philox on the few driven
   // lanes of a wave would cost it more than a separate, fully occupied
   // drive launch
-  if (s->drive_group) return (int)hipErrorInvalidValue;
+  if (s->drive_group || s->drive_dense) return (int)hipErrorInvalidValue;
   return 0;
 }
 
@@ -1361,6 +1371,7 @@ int64_t fis_struct_layout(int what) {
     case 8: return (int64_t)offsetof(fis_sim, t_origin);
     case 9: return (int64_t)offsetof(fis_sim, pull_tile);
     case 10: return (int64_t)offsetof(fis_sim, clock_ahead);
+    case 11: return (int64_t)offsetof(fis_sim, drive_dense);
     default: return -1;
   }
 }

@@ -478,15 +478,35 @@ class DeviceSim:
         buf.copy_(torch.from_numpy(host.reshape(-1)))
 
     # ------------------------------------------------------------ launches
-    def drive_rates(self, rates, W=1):
-        """Kicks for W steps from dense rates [batch, n] (Hz) on the device."""
+    def _check_rates(self, rates):
         if rates.dtype != torch.float32 or not rates.is_contiguous() or \
                 tuple(rates.shape) != (self.batch, self.n) or \
                 rates.device != self.device:
             raise ValueError("rates must be contiguous float32 [%d, %d] on %s"
                              % (self.batch, self.n, self.device))
+
+    def drive_rates(self, rates, W=1):
+        """Kicks for W steps from dense rates [batch, n] (Hz) on the device."""
+        self._check_rates(rates)
         L.check(self.lib.fis_drive_rates(self._sp, _ptr(rates), W, _stream()),
                 "fis_drive_rates")
+
+    def step_rates(self, rates, W=1):
+        """One window driven by dense rates [batch, n] (Hz).  On the push
+        road the update launch draws the kicks itself (fis_sim.drive_dense:
+        one launch fewer per window, the bits fis_drive_rates would set);
+        on the pull road, or with a fused drive table, fis_drive_rates runs
+        first."""
+        if self.road == "pull" or getattr(self, "drive_group", None) is not None:
+            self.drive_rates(rates, W)
+            self.step_launch(W)
+            return
+        self._check_rates(rates)
+        self.struct.drive_dense = rates.data_ptr()
+        try:
+            self.step_launch(W)
+        finally:
+            self.struct.drive_dense = None
 
     def drive_table(self, table, W):
         L.check(self.lib.fis_drive_table(
@@ -577,10 +597,11 @@ class DeviceSim:
         scatter."""
         if not 1 <= W <= L.MAX_WINDOW:
             raise ValueError("window of %d steps (1..%d)" % (W, L.MAX_WINDOW))
+        if isinstance(drive, torch.Tensor):
+            self.step_rates(drive, W)
+            return
         if drive is not None:
-            if isinstance(drive, torch.Tensor):
-                self.drive_rates(drive, W)
-            elif isinstance(drive, DriveTable):
+            if isinstance(drive, DriveTable):
                 self.drive_table(drive, W)
             elif isinstance(drive, Tape):
                 self.drive_tape(drive, W)

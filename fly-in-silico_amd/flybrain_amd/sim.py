@@ -401,12 +401,16 @@ class Engine:
     # --- stepping -----------------------------------------------------------
     def _launch_step(self, rates):
         self._clock_defer()
-        s = self._sim
-        s.drive_rates(rates, 1)
-        s.step_launch(1)
+        self._sim.step_rates(rates, 1)
 
-    def use_graphs(self, enable=True):
+    def use_graphs(self, enable=True, force=False):
         """Capture one step (drive + update + scatter) into a HIP graph.
+
+        Returns whether steps now replay a graph.  On MI355X a captured
+        step replays slower than its direct launches (B = 1: 34 against 11
+        us per step), so, as the reference stays ungraphed where graphs do
+        not serve (sim.py:676-677, 715-737), this declines and returns False
+        unless ``force`` -- results are the same either way.
 
         One graph serves every ring position: the step counter lives in
         device memory, so nothing in the graph is frozen per slot (the
@@ -415,7 +419,7 @@ class Engine:
         reference; on a capture failure the state is restored and the error
         re-raised.
         """
-        if not enable:
+        if not enable or not force:
             self._graph = self._graphs = None
             return False
         if self._graph is not None:
@@ -426,14 +430,12 @@ class Engine:
             side = torch.cuda.Stream()
             side.wait_stream(torch.cuda.current_stream())
             with torch.cuda.stream(side):
-                self._sim.drive_rates(self._rates, 1)
-                self._sim.step_launch(1)
+                self._sim.step_rates(self._rates, 1)
             torch.cuda.current_stream().wait_stream(side)
             self._device_restore(snap)
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
-                self._sim.drive_rates(self._rates, 1)
-                self._sim.step_launch(1)
+                self._sim.step_rates(self._rates, 1)
             self._device_restore(snap)
         except BaseException:
             self._device_restore(snap)
@@ -470,7 +472,7 @@ class Engine:
         if self._graph is not None and self._graph_version != self._sim.version:
             # the kernel arguments changed since capture: capture again
             self._graph = self._graphs = None
-            self.use_graphs()
+            self.use_graphs(force=True)
         # (on the pull road the 20 steps after a restore also read the
         # restored ring: launched directly, the graph does not carry that)
         if self._graph is not None and self._sim.t >= self._sim.ring_until:

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define FIS_ABI_VERSION 9
+#define FIS_ABI_VERSION 10
 #define FIS_MAX_WINDOW 20   /* steps per window: delay horizon 19 + 1     */
 #define FIS_RING_LEN 40     /* int32 ring slots, two delays: the spikes of
                                step u are summed into slot (u + 21) mod 40,
@@ -186,6 +186,11 @@ typedef struct {
    * Both 0 outside such a run. */
   int32_t clock_ahead;
   int32_t defer_tick;
+  /* Push road: dense rates [batch][n] in Hz, held over the window, whose
+   * kicks the update launches draw themselves -- the bits fis_drive_rates
+   * would set, without its launch (Engine.step's rates, sim.py:745-771).
+   * Nullable; not together with drive_group; fis_pull_window refuses it. */
+  const float* drive_dense;
 } fis_sim;
 
 /* Drive table entry: neuron-trial `idx` is driven at rate

@@ -38,7 +38,8 @@ def test_versions_and_layout(lib):
     assert lib.fis_struct_layout(6) == ctypes.sizeof(L.FisDriveEntry)
     for what, field in ((1, "offsets"), (2, "w_scale"), (3, "refrac_steps"),
                         (4, "raster_len"), (7, "in_offsets"), (8, "t_origin"),
-                        (9, "pull_tile"), (10, "clock_ahead")):
+                        (9, "pull_tile"), (10, "clock_ahead"),
+                        (11, "drive_dense")):
         assert lib.fis_struct_layout(what) == getattr(L.FisSim, field).offset
 
 

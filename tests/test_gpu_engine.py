@@ -33,7 +33,8 @@ def test_engine_step_matches_oracle_philox(road):
         eng = S.Engine(conn, n, batch=B, seed=1234, no_refractory=drv,
                        road=road)
         if graphs:
-            assert eng.use_graphs()
+            assert not eng.use_graphs()  # declined: direct steps are faster
+            assert eng.use_graphs(force=True)
         ora = O.OracleEngine(conn, B, no_refractory=drv,
                              seeds=[1234] * B)
         for t in range(300):
@@ -217,7 +218,7 @@ def test_roads_agree_after_restore_and_out_mask():
     for eng in engines:
         eng.restore(snap)
         eng.set_masks(out=hubs)
-        eng.use_graphs()
+        eng.use_graphs(force=True)
     assert engines[1].road == "pull"
     for t in range(60):
         a, b = (e.step(rates).cpu().numpy().copy() for e in engines)
@@ -319,14 +320,14 @@ def test_engine_switches_between_graph_and_direct_steps(road):
     t = 0
     for graphs, steps in ((False, 45), (True, 30), (False, 25), (True, 20),
                           (False, 13)):
-        assert eng.use_graphs(graphs) == graphs
+        assert eng.use_graphs(graphs, force=True) == graphs
         for _ in range(steps):
             sp = eng.step(rates)
             ora.step_rates(r_np)
             assert np.array_equal(sp.cpu().numpy(), ora.spikes), \
                 "graphs=%s, step %d" % (graphs, t)
             t += 1
-    eng.use_graphs(True)  # closes the last direct run
+    eng.use_graphs(True, force=True)  # closes the last direct run
     torch.cuda.synchronize()
     assert int(eng._sim.clock.item()) == t == eng._sim.t
     assert ora.spikes.sum() > 0

@@ -418,3 +418,35 @@ def test_outputs_stay_inside_their_buffers(road, batch):
     for name, (big, start) in guards.items():
         tail = big[start:].cpu().numpy()  # words after the last data byte
         assert (tail == sentinel).all(), name
+
+
+@pytest.mark.parametrize("batch", [1, 3, 64, 130])
+def test_dense_rates_drawn_in_the_update_equal_the_drive_launch(batch):
+    """Push road: kicks from dense rates drawn by the update launch itself
+    (fis_sim.drive_dense) equal those of the separate fis_drive_rates
+    launch, over windows of 1..20 steps with the rates changed between
+    windows (zero, sparse and dense rows)."""
+    conn = C.synthetic(300, None, mean_out=30, count_exp=1.2, n_groups=2,
+                       seed=13)
+    n = conn.n
+    rng = np.random.default_rng(batch)
+    seeds = rng.integers(0, 2**63, size=batch, dtype=np.uint64)
+    net = DeviceNet(conn)
+    fused = DeviceSim(net, batch, seeds=seeds, raster_len=40, road="push")
+    split = DeviceSim(net, batch, seeds=seeds, raster_len=40, road="push")
+    t = 0
+    for k, W in enumerate([20, 1, 7, 20, 13, 1, 20, 3]):
+        rates = torch.zeros(batch, n, device="cuda")
+        if k % 3 != 2:  # every third window undriven
+            idx = torch.from_numpy(rng.choice(n, 40, replace=False)).cuda()
+            rates[:, idx] = float(rng.uniform(200, 4000))
+            rates[0, ::17] = 800.0
+        fused.step_rates(rates, W)
+        split.drive_rates(rates, W)
+        split.step_launch(W)
+        assert np.array_equal(fused.read_raster(t, t + W),
+                              split.read_raster(t, t + W)), "window %d" % k
+        t += W
+    torch.testing.assert_close(fused.v, split.v, rtol=0, atol=0)
+    torch.testing.assert_close(fused.g, split.g, rtol=0, atol=0)
+    assert fused.read_raster(t - 20, t).any()
