@@ -116,6 +116,28 @@ class Result:
     def active(self, min_rate=0.5):
         return np.nonzero(self.rates >= min_rate)[0]
 
+    # the reference's neuron-table columns of a rate table (sim.py:286-289)
+    NEURON_COLUMNS = ("root_id", "label", "cell_class", "super_class",
+                      "region", "super_region", "side", "top_nt")
+
+    def rate_table(self, min_rate=0.5):
+        """The active neurons as a DataFrame indexed by neuron, by rate
+        descending, with ``rate_hz`` and ``n_spikes`` (sim.py:282-292).
+        The FlyWire neuron-table columns come first where ``neurons`` is
+        such a table; with a plain index array there are none."""
+        import pandas as pd
+
+        idx = self.active(min_rate)
+        nt = self.neurons
+        if isinstance(nt, pd.DataFrame) and all(
+                c in nt.columns for c in self.NEURON_COLUMNS):
+            df = nt.loc[idx, list(self.NEURON_COLUMNS)].copy()
+        else:
+            df = pd.DataFrame(index=pd.Index(idx))
+        df["rate_hz"] = self.rates[idx]
+        df["n_spikes"] = self.counts[idx]
+        return df.sort_values("rate_hz", ascending=False)
+
     def meta(self, with_indices=False):
         return {
             "config": {k: v for k, v in asdict(self.config).items()
@@ -800,12 +822,13 @@ def _drive_for(sim, rates):
 
 
 def save(result: Result, out_dir, min_rate=0.5):
-    """Write meta.json and bins.npz into ``out_dir`` (sim.py:1120-1140).
+    """Write meta.json, rates.parquet and bins.npz into ``out_dir``, the
+    reference's files (sim.py:1120-1140): the reference's ``load`` reads
+    them back.
 
+    ``rates.parquet`` is :meth:`Result.rate_table` (index = neuron).
     ``bins.npz`` has the reference's keys: ``neuron_idx`` int32 (active
     neurons), ``bins`` int16 [n_bins, n_active], ``bin_ms``, ``t0_ms``.
-    ``rates.npz`` replaces the reference's rates.parquet (which needs the
-    FlyWire neuron table): ``neuron_idx``, ``rate_hz``, ``n_spikes``.
     """
     out_dir = Path(out_dir)
     out_dir.mkdir(parents=True, exist_ok=True)
@@ -813,8 +836,7 @@ def save(result: Result, out_dir, min_rate=0.5):
         json.dumps(result.meta(with_indices=True), ensure_ascii=False,
                    indent=2, default=_json_default), encoding="utf-8")
     keep = result.active(min_rate)
-    np.savez(out_dir / "rates.npz", neuron_idx=keep.astype(np.int32),
-             rate_hz=result.rates[keep], n_spikes=result.counts[keep])
+    result.rate_table(min_rate).to_parquet(out_dir / "rates.parquet")
     np.savez_compressed(
         out_dir / "bins.npz", neuron_idx=keep.astype(np.int32),
         bins=result.bins[:, keep].astype(np.int16),
@@ -830,9 +852,15 @@ def load(out_dir, neurons=None) -> Result:
     n = meta["n_neurons"] if neurons is None else len(neurons)
     if neurons is None:
         neurons = np.arange(n)
-    r = np.load(out_dir / "rates.npz", allow_pickle=False)
     counts = np.zeros(n, dtype=np.int64)
-    counts[r["neuron_idx"]] = r["n_spikes"]
+    if (out_dir / "rates.parquet").exists():
+        import pandas as pd
+
+        rates_df = pd.read_parquet(out_dir / "rates.parquet")
+        counts[rates_df.index.to_numpy()] = rates_df["n_spikes"].to_numpy()
+    else:  # runs saved by an earlier version of this engine
+        r = np.load(out_dir / "rates.npz", allow_pickle=False)
+        counts[r["neuron_idx"]] = r["n_spikes"]
     z = np.load(out_dir / "bins.npz", allow_pickle=False)
     bins = np.zeros((z["bins"].shape[0], n), dtype=np.int16)
     bins[:, z["neuron_idx"]] = z["bins"]
