@@ -224,6 +224,48 @@ def from_reference_coo(obj, n=None) -> Connectome:
     return from_edges(n, idx[1], idx[0], val)
 
 
+# FlyWire v783 tables as the reference reads them (lab/flybrain/data.py:
+# 41-42, 239-252): the connectivity parquet's index and value columns, and
+# the completeness CSV, whose row count is N.
+FLYWIRE_COLUMNS = ("Presynaptic_Index", "Postsynaptic_Index",
+                   "Excitatory x Connectivity")
+
+
+def from_flywire_tables(conn_parquet, completeness_csv=None, n=None
+                        ) -> Connectome:
+    """Build the connectome from the reference's FlyWire files: edges from
+    the connectivity parquet's three columns above (duplicates summed, as
+    the reference's ``coalesce()``), N from the completeness CSV's rows
+    (``pd.read_csv(..., index_col=0)``, data.py:239-240) or ``n``."""
+    import pandas as pd
+
+    if n is None:
+        if completeness_csv is None:
+            raise ValueError("N comes from the completeness CSV or n=")
+        n = len(pd.read_csv(completeness_csv, index_col=0))
+    df = pd.read_parquet(conn_parquet, columns=list(FLYWIRE_COLUMNS))
+    return from_edges(int(n), df[FLYWIRE_COLUMNS[0]].to_numpy(np.int64),
+                      df[FLYWIRE_COLUMNS[1]].to_numpy(np.int64),
+                      df[FLYWIRE_COLUMNS[2]].to_numpy(np.float32))
+
+
+def to_flywire_tables(conn: Connectome, conn_parquet, completeness_csv=None):
+    """Write ``conn`` in the reference's FlyWire table layout (the three
+    connectivity columns; a completeness CSV of N rows indexed by neuron),
+    so a synthetic connectome goes through the reference's loader."""
+    import pandas as pd
+
+    pre = np.repeat(np.arange(conn.n, dtype=np.int64), conn.degree())
+    pd.DataFrame({FLYWIRE_COLUMNS[0]: pre,
+                  FLYWIRE_COLUMNS[1]: conn.cols.astype(np.int64),
+                  FLYWIRE_COLUMNS[2]: conn.vals.astype(np.int64)}).to_parquet(
+        conn_parquet, index=False)
+    if completeness_csv is not None:
+        pd.DataFrame({"Completed": np.ones(conn.n, bool)},
+                     index=pd.Index(np.arange(conn.n), name="root_id")
+                     ).to_csv(completeness_csv)
+
+
 def save(conn: Connectome, path):
     """Write the by-pre CSR as an uncompressed .npz."""
     extra = {} if conn.groups is None else {"groups": conn.groups}

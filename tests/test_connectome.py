@@ -163,3 +163,28 @@ def test_pull_layout_lists_every_in_edge_by_position(bstride, locality):
         np.add.at(got, cols[off[k]:off[k + 1]], vals[off[k]:off[k + 1]])
         assert np.array_equal(got, dense[order[k]]), k
     assert torch.is_tensor(net.pull_layout(bstride)[1])
+
+
+def test_flywire_tables_round_trip(tmp_path):
+    """The reference's FlyWire layout (connectivity parquet with the index
+    and 'Excitatory x Connectivity' columns, completeness CSV giving N)
+    round-trips, and duplicate rows are summed as coalesce() sums them."""
+    import pandas as pd
+
+    c = C.synthetic(400, None, mean_out=20, n_groups=0, seed=4)
+    C.to_flywire_tables(c, tmp_path / "conn.parquet", tmp_path / "comp.csv")
+    d = C.from_flywire_tables(tmp_path / "conn.parquet", tmp_path / "comp.csv")
+    assert d.n == c.n and d.fingerprint() == c.fingerprint()
+    df = pd.read_parquet(tmp_path / "conn.parquet")
+    assert list(df.columns) == list(C.FLYWIRE_COLUMNS)
+    # a duplicated connection row: its counts add up
+    dup = pd.concat([df, df.iloc[:5]], ignore_index=True)
+    dup.to_parquet(tmp_path / "dup.parquet", index=False)
+    e = C.from_flywire_tables(tmp_path / "dup.parquet", n=c.n)
+    pre = np.repeat(np.arange(c.n), c.degree())
+    for k in range(5):
+        i, j = int(df.iloc[k, 0]), int(df.iloc[k, 1])
+        row = slice(e.offsets[i], e.offsets[i + 1])
+        got = e.vals[row][e.cols[row] == j]
+        assert got.tolist() == [2 * int(df.iloc[k, 2])]
+    assert e.n_edges == c.n_edges and pre.size == c.n_edges
