@@ -654,8 +654,10 @@ def build_engine(weights=None, neurons=None, silenced=None, params=None,
     a ``{indices, values, shape}`` dict, or a Connectome."""
     if weights is None:
         raise ValueError("no connectome given: the FlyWire tables are not part "
-                         "of this engine; pass weights (see "
-                         "flybrain_amd.connectome)")
+                         "of this engine; pass weights, e.g. "
+                         "flybrain_amd.connectome.from_flywire_tables("
+                         "'2025_Connectivity_783.parquet', "
+                         "'2025_Completeness_783.csv')")
     if neurons is None:
         n = weights.n if isinstance(weights, C.Connectome) else \
             int((weights["shape"] if isinstance(weights, dict)
