@@ -510,6 +510,45 @@ __device__ __forceinline__ uint32_t run_window_plain(const fis_sim& s, Cell& c,
   return fired;
 }
 
+// A full window of lif_step for a wave with no fire mask off and no hold
+// (the push road's common case, also with driven neurons): as
+// run_window_plain, with the Poisson kick term kept and the gate threshold
+// per lane (refrac_steps, 0 without refractoriness, INT_MAX with the input
+// masked: R - k does not overflow, and b never reaches it).  Equal in
+// every value to run_window; returns the steps fired (bit k = step k).
+template <int WMAX>
+__device__ __forceinline__ uint32_t run_window_kick(const fis_sim& s, Cell& c,
+                                                    const int32_t (&in)[WMAX],
+                                                    uint32_t kicks, Stim stim) {
+  asm volatile("" : "+v"(stim.on), "+v"(stim.off));
+  int b = c.sp ? 0 : c.r + 1;  // counter after the update of step 0
+  float v = c.v, g = c.g;
+  bool fire = false;
+  uint32_t fired = 0;
+#pragma unroll
+  for (int k = 0; k < WMAX; ++k) {
+    const float rec = s.w_scale * (float)in[k];
+    const float gd = g * s.a_syn;
+    const float g_new = (b >= c.R - k) ? gd + rec : gd;
+    float vv = v + (((kicks >> k) & 1u) ? stim.on : stim.off);
+    const float t1 = vv - s.v_rest;
+    const float t2 = g - t1;
+    const float t3 = s.a_mem * t2;
+    vv = vv + t3;
+    fire = vv > s.v_th;
+    const float t4 = vv - s.v_reset;
+    v = fire ? vv - t4 : vv;
+    g = fire ? 0.0f : g_new;
+    fired |= fire ? (1u << k) : 0u;
+    if (k + 1 < WMAX) b = fire ? -(k + 1) : b;
+  }
+  c.v = v;
+  c.g = g;
+  c.sp = fire;
+  c.r = (WMAX - 1) + b;  // the counter after the last update
+  return fired;
+}
+
 // One event's CSR row, read once and added into every trial of its mask.
 __device__ __forceinline__ void scatter_event(const fis_sim& s,
                                               const fis_event& ev, int lane) {
@@ -622,9 +661,15 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
     }
   }
 
-  // bit k: this lane fired at step k
+  // bit k: this lane fired at step k.  Waves with no fire mask off and no
+  // hold in a full window take the kick recurrence.
+  const bool kick_path =
+      W == WMAX &&
+      !any_lane(act && (fl & (FIS_FIRE_OFF | FIS_HOLD)) != 0u);
   const uint32_t fired_bits =
-      act ? run_window<WMAX>(s, c, in, sb, stim, W) : 0u;
+      !act ? 0u
+           : (kick_path ? run_window_kick<WMAX>(s, c, in, sb, stim)
+                        : run_window<WMAX>(s, c, in, sb, stim, W));
   const int64_t left = t_stop - t0;  // steps whose spikes still count
   const int kstop = left <= 0 ? 0 : (left >= W ? W : (int)left);
   const uint32_t nspk = __popc(fired_bits & ((1u << kstop) - 1u));
