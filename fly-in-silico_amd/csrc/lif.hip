@@ -594,6 +594,36 @@ __device__ __forceinline__ void scatter_leads(const fis_sim& s, bool lead,
   }
 }
 
+// One trial per lane (bstride 1): the CSR row of every lane with a set
+// bit in `sent` (bit k: the lane's neuron sent at step t0 + k), read once
+// by the whole wave, added into ring slot (t0 + k + 21) mod 40 for each
+// such step (integer sums: the order of the adds does not matter).
+__device__ __forceinline__ void scatter_row_steps(const fis_sim& s,
+                                                  uint32_t sent, int pre,
+                                                  int64_t t0, int lane) {
+  const int64_t nb = (int64_t)s.n;  // ring slot stride at bstride 1
+  const int slot0 = (int)((t0 + kDelay + 1) % FIS_RING_LEN);
+  uint64_t leads = __ballot(sent != 0u);
+  asm volatile("s_mov_b64 %0, %1" : "=s"(leads) : "s"(leads));
+  while (leads) {
+    const int l = __ffsll((long long)leads) - 1;
+    leads &= leads - 1ull;
+    const int pl = __builtin_amdgcn_readlane(pre, l);
+    const uint32_t steps = (uint32_t)__builtin_amdgcn_readlane((int)sent, l);
+    const int64_t beg = s.offsets[pl], end = s.offsets[pl + 1];
+    for (int64_t e = beg + lane; e < end; e += kWave) {
+      const int32_t post = s.cols[e];
+      const int32_t val = s.vals[e];
+      for (uint32_t x = steps; x; x &= x - 1u) {  // uniform over the wave
+        int sl = slot0 + __ffs(x) - 1;
+        sl = sl >= FIS_RING_LEN ? sl - FIS_RING_LEN : sl;
+        __hip_atomic_fetch_add(s.ring + (int64_t)sl * nb + post, val,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  }
+}
+
 // W steps per neuron-trial.  WMAX is the unrolled bound (1 or 20).
 // Index arithmetic is 32-bit: n * bstride < 2^31 is checked on the host.
 template <int WMAX, bool FUSED>
@@ -693,13 +723,17 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
           rrow = (rrow + 1 == s.raster_len) ? 0 : rrow + 1;
         }
         const bool snd = (sent >> k) & 1u;
-        if (any_sent && any_lane(snd)) {
+        if (bs > 1 && any_sent && any_lane(snd)) {
           // the trials of each neuron (bs lanes from trial 0) that sent
           const uint64_t sub = group_or(snd ? (1ull << tb) : 0ull, bs);
           scatter_leads(s, tb == 0 && sub != 0ull, pre, sub, slot, lane);
         }
         slot = (slot + 1 == FIS_RING_LEN) ? 0 : slot + 1;
       }
+      // one trial per lane: each neuron that sent reads its CSR row once
+      // and adds it into the slot of every step it sent at (a neuron
+      // spiking twice in the window walks its row once, not twice)
+      if (bs == 1 && any_sent) scatter_row_steps(s, sent, pre, t0, lane);
     }
     return;  // the clock moves on in k_tick, after every workgroup read it
   }
