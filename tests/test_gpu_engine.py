@@ -331,3 +331,24 @@ def test_engine_switches_between_graph_and_direct_steps(road):
     torch.cuda.synchronize()
     assert int(eng._sim.clock.item()) == t == eng._sim.t
     assert ora.spikes.sum() > 0
+
+
+@pytest.mark.parametrize("road", ["push", "pull"])
+def test_rate_cap_and_zero_rates_match_oracle(road):
+    """Rates at the cap (10,000 Hz: a kick every step, p = 1.0f) and
+    exactly zero, side by side in one batch: every step equals the
+    oracle's, and a capped neuron (refractory-exempt) fires every step."""
+    conn = C.synthetic(300, None, mean_out=30, count_exp=1.2, n_groups=2,
+                       seed=17)
+    n, B = conn.n, 2
+    drv = np.arange(0, n, 13)
+    rates = torch.zeros(B, n, device="cuda")
+    rates[0, drv] = 10_000.0  # trial 1 stays undriven
+    eng = S.Engine(conn, n, batch=B, seed=99, no_refractory=drv, road=road)
+    ora = O.OracleEngine(conn, B, no_refractory=drv, seeds=[99] * B)
+    r_np = rates.cpu().numpy()
+    for t in range(80):
+        sp = eng.step(rates).cpu().numpy()
+        ora.step_rates(r_np)
+        assert np.array_equal(sp, ora.spikes), "step %d" % t
+        assert sp[0, drv].all(), "a capped neuron missed step %d" % t
