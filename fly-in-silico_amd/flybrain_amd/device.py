@@ -29,6 +29,11 @@ NO_END = 1 << 62  # t_end of a trial that never stops counting
 PULL_MIN_BATCH = 64
 
 
+def _nonempty(a):
+    """``a``, or one zero of its dtype when it is empty."""
+    return a if a.size else np.zeros(1, a.dtype)
+
+
 def pick_road(batch: int, road: str = "auto") -> str:
     """``auto``: the pull road from 64 trials up (to 1,024), else push."""
     if road == "auto":
@@ -78,8 +83,10 @@ class DeviceNet:
             dev = torch.device("cuda", torch.cuda.current_device())
         self.device = dev
         self.offsets = torch.from_numpy(conn.offsets).to(self.device)
-        self.cols = torch.from_numpy(conn.cols).to(self.device)
-        self.vals = torch.from_numpy(conn.vals).to(self.device)
+        # edge arrays of at least one element: a network without edges
+        # still passes the library valid (never read) pointers
+        self.cols = torch.from_numpy(_nonempty(conn.cols)).to(self.device)
+        self.vals = torch.from_numpy(_nonempty(conn.vals)).to(self.device)
         self._layouts = {}
 
     locality = True  # pull road: order neurons by group (see pull_layout)
@@ -128,8 +135,8 @@ class DeviceNet:
         dev = self.device
         got = (torch.from_numpy(order.astype(np.int32)).to(dev),
                torch.from_numpy(off.astype(np.int32)).to(dev),
-               torch.from_numpy(pre[take]).to(dev),
-               torch.from_numpy(c.vals[take]).to(dev))
+               torch.from_numpy(_nonempty(pre[take])).to(dev),
+               torch.from_numpy(_nonempty(c.vals[take])).to(dev))
         self._layouts[key] = got
         return got
 

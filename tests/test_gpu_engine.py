@@ -352,3 +352,23 @@ def test_rate_cap_and_zero_rates_match_oracle(road):
         ora.step_rates(r_np)
         assert np.array_equal(sp, ora.spikes), "step %d" % t
         assert sp[0, drv].all(), "a capped neuron missed step %d" % t
+
+
+@pytest.mark.parametrize("road", ["push", "pull"])
+def test_network_without_edges_runs(road):
+    """Every neuron silenced, or no synapse at all: the engine runs with
+    zero recurrent input, as the reference's empty W does; driven neurons
+    follow their kicks exactly as the oracle's."""
+    conn = C.from_edges(120, [], [], [])
+    n, B = conn.n, 2
+    drv = np.arange(0, n, 7)
+    rates = torch.zeros(B, n, device="cuda")
+    rates[:, drv] = 2000.0
+    eng = S.Engine(conn, n, batch=B, seed=3, no_refractory=drv, road=road)
+    ora = O.OracleEngine(conn, B, no_refractory=drv, seeds=[3] * B)
+    r_np = rates.cpu().numpy()
+    for t in range(60):
+        sp = eng.step(rates).cpu().numpy()
+        ora.step_rates(r_np)
+        assert np.array_equal(sp, ora.spikes), "step %d" % t
+    assert ora.spikes.sum() >= 0 and sp[:, np.setdiff1d(np.arange(n), drv)].sum() == 0
