@@ -74,6 +74,13 @@ def kernel_times(runner, windows):
                    ("scatter", lambda: s.scatter())]
     ev = {k: [] for k, _ in stages}
     stream = torch.cuda.current_stream()
+    # untimed windows first: after the host-side pause that follows the
+    # timed region the first eager windows run slow (956 -> 853 us over 20
+    # windows of k_pull at B = 256, in a kernel trace) while the GPU ramps
+    # back up
+    for _ in range(windows // 2):
+        for _, fn in stages:
+            fn()
     for _ in range(windows):
         marks = [torch.cuda.Event(enable_timing=True)
                  for _ in range(len(stages) + 1)]
@@ -220,11 +227,19 @@ def summarize(paths):
             "max(duration), sum(duration) from kernels group by name, grid_x "
             "order by sum(duration) desc").fetchall()
         if rows and rows[0][2]:
-            print("%-60s %10s %7s %12s %12s %12s" % (
-                "kernel", "grid", "calls", "avg_us", "min_us", "max_us"))
+            # the median too: a run's quieter first windows (state from
+            # rest) pull the mean below the steady-state duration
+            print("%-60s %10s %7s %12s %12s %12s %12s" % (
+                "kernel", "grid", "calls", "avg_us", "median_us", "min_us",
+                "max_us"))
             for name, gx, n, avg, mn, mx, tot in rows[:14]:
-                print("%-60s %10d %7d %12.3f %12.3f %12.3f" % (
-                    _short(name), gx, n, avg / 1e3, mn / 1e3, mx / 1e3))
+                durs = sorted(d for (d,) in con.execute(
+                    "select duration from kernels where name = ? and "
+                    "grid_x = ?", (name, gx)))
+                med = durs[len(durs) // 2] if durs else 0.0
+                print("%-60s %10d %7d %12.3f %12.3f %12.3f %12.3f" % (
+                    _short(name), gx, n, avg / 1e3, med / 1e3, mn / 1e3,
+                    mx / 1e3))
         pmc = con.execute(
             "select kernel_name, grid_size, counter_name, count(*), avg(value) "
             "from counters_collection group by kernel_name, grid_size, "
