@@ -1,23 +1,33 @@
 """Benchmark: simulated trial-seconds per wall-second on one MI355X.
 
 Workloads (138,639 neurons, 15,091,983 edges, synthetic, seeded):
-  headline  sensory_b256: B = 256 trials, four sensory index sets driven at
-            piecewise-constant U(0, 300) Hz rates (10 ms segments), per-trial
-            m_out masks, per-trial Philox seeds; windows of 20 steps replayed
-            from one captured HIP graph
-  b1        plain_b1: B = 1, 20 neurons at 150 Hz -- the plain engine
-  b64       trials_b64: B = 64 trials, each driving its own 20-200 neurons
-            at U(50, 250) Hz with its own Philox seed; road and pull-road
-            workgroup shape (trial-block width) autotuned
+  headline    sensory_b256: B = 256 trials, four sensory index sets driven
+              at piecewise-constant U(0, 300) Hz rates (10 ms segments),
+              per-trial m_out masks, per-trial Philox seeds
+  b1          plain_b1: B = 1, 20 neurons at 150 Hz -- the plain engine
+  plain_b256  the plain_b1 drive replicated over B = 256 trials with
+              distinct seeds: the like-for-like B = 256 : B = 1 ratio
+              (every trial is the plain workload; batched = alone is a GPU
+              test)
+  b64         trials_b64: B = 64 trials, each driving its own 20-200
+              neurons at U(50, 250) Hz with its own Philox seed
+
+Every leg first runs ``--preroll`` untimed steps from rest (working
+activity; independent of ``--warmup``), then ``--warmup`` untimed steps,
+then the timed steps.  Windows are 20 steps; a HIP graph of
+min(``--graph-windows``, steps // 20) windows is captured, so the timed
+steps are graph replays (plus eager windows for a remainder).
 
 A bench "step" is one 0.1 ms model step of every trial.  ``value`` is
 B * steps * 1e-4 / wall seconds of the timed region (inputs and state
-resident in HBM).  The JSON line also carries the B = 1 and B = 64 rates,
-the B = 256 : B = 1 ratio,
-the roofline of the dominant kernel (per-launch durations from HIP events
-on the launch stream) and a CPU baseline: the oracle's torch restatement of
-the reference step (csrmv via torch.sparse.mm) on this host's cores, over a
-bounded sample of the B = 1 workload.
+resident in HBM).  ``roofline`` is the dominant kernel's: ``achieved`` is
+the per-window byte model (``window_bytes``: what one launch must move)
+over the launch's average duration from HIP events on its stream;
+``traffic`` is the HBM bytes per launch rocprofv3 counted (FETCH_SIZE and
+WRITE_SIZE, committed under profiles/); ``valu_issue_frac`` is the
+profiled VALU instructions per launch over what the chip issues in that
+time.  ``cpu_baseline`` times the oracle's torch restatement of the
+reference step (csrmv via torch.sparse.mm) on this host's cores.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -37,7 +47,16 @@ for _p in (ROOT / "fly-in-silico_amd", ROOT / "oracle"):
         sys.path.insert(0, str(_p))
 
 HBM_PEAK_GBS = 8000.0
-BETA = 26  # algorithmic bytes per neuron-trial-step of the dense update
+# wave64 VALU instructions the chip can issue per second: 256 CUs x 4 SIMDs
+# x 2.4 GHz / 2 cycles per wave64 instruction (the 157.3 TFLOP/s fp32 peak
+# as 2-flop FMAs over 64 lanes)
+VALU_ISSUE_PER_S = 1.2288e12
+# the scope table's dense term: bytes per neuron-trial-step of the update
+# priced with a state round trip every step (reported as effective_gbs)
+BETA = 26
+# instructions of the plain recurrence per neuron-trial-step (gfx950
+# assembly of run_window_plain)
+PLAIN_STEP_VALU = 18
 
 
 def log(*a):
@@ -52,31 +71,35 @@ def timed(fn, sync):
     return time.perf_counter() - t0
 
 
-def kernel_times(runner, windows):
-    """Average durations (ms) of the drive launch and of the road's
-    launches -- push: update, scatter (one fused launch below 64 trials);
-    pull: the fused window kernel --
-    over ``windows`` eager windows, bracketed by HIP events on the launch
-    stream."""
-    import torch
-
+def stages_of(runner):
+    """(name, launch) of one window of ``runner``'s road: the drive launch
+    (unless fused) and push update + scatter (one fused launch below 64
+    trials) or the pull road's fused window kernel."""
     s = runner.sim
     W = runner.window
-    stages = [] if runner.fused else [
+    st = [] if runner.fused else [
         ("drive", lambda: runner.table is not None
          and s.drive_table(runner.table, W))]
     if s.road == "pull":
-        stages.append(("pull", lambda: s.pull(W)))
+        st.append(("pull", lambda: s.pull(W)))
     elif s.bstride < 64:
-        stages.append(("update_scatter", lambda: s.update_scatter(W)))
+        st.append(("update_scatter", lambda: s.update_scatter(W)))
     else:
-        stages += [("update", lambda: s.update(W)),
-                   ("scatter", lambda: s.scatter())]
+        st += [("update", lambda: s.update(W)),
+               ("scatter", lambda: s.scatter())]
+    return st
+
+
+def kernel_times(runner, windows):
+    """Average durations (ms) of each launch of a window over ``windows``
+    eager windows, bracketed by HIP events on the launch stream."""
+    import torch
+
+    stages = stages_of(runner)
     ev = {k: [] for k, _ in stages}
     stream = torch.cuda.current_stream()
     # untimed windows first: after the host-side pause that follows the
-    # timed region the first eager windows run slow (956 -> 853 us over 20
-    # windows of k_pull at B = 256, in a kernel trace) while the GPU ramps
+    # timed region the first eager windows run slow while the GPU ramps
     # back up
     for _ in range(windows // 2):
         for _, fn in stages:
@@ -94,23 +117,123 @@ def kernel_times(runner, windows):
             for k, v in ev.items()}
 
 
-def scatter_bytes(conn, raster_rows):
-    """Algorithmic bytes of the scatter for given spikes: per spiking
-    (neuron, step) row 8 B offsets + 6 B per edge, plus an 8 B int32
-    read-modify-write per edge and spiking trial."""
+def popcount64(a):
+    import numpy as np
+    return np.bitwise_count(np.ascontiguousarray(a).view(np.uint64))
+
+
+def pull_window_bytes(sim, conn, W):
+    """Bytes one pull-road window of ``W`` steps starting now must move
+    (what the kernel reads and writes once per window; quiet neuron-trials
+    are priced like the others):
+
+      state       23 B per neuron-trial: v, g read + written (16), the
+                  refractory byte read + written (2), flags (1), the
+                  window's kick bits (4)
+      in-edges    per tile part: 4 B source + 2 B count + 8 B act word per
+                  in-edge, 4 B offset per neuron
+      records     8 B per 64-trial word of the tile part, per active
+                  (in-edge, horizon step) pair -- a step at which the
+                  source sent in some trial of the part
+      own records W steps x bstride / 64 words x 8 B per neuron written,
+                  and its act word read-modify-written (16 B per part)
+      counts      8 B per neuron-trial that spiked in the window
+
+    The active pairs are counted from the act words the window reads; the
+    spiking neuron-trials from the records it writes (the window is run).
+    """
     import numpy as np
 
-    deg = conn.degree()
-    rows = 0
-    edges_trials = 0
-    for sp in raster_rows:  # bool [batch, n] per step
-        any_t = sp.any(axis=0)
-        rows += int((8 + 6 * deg[any_t]).sum())
-        edges_trials += int((deg[None, :] * sp).sum())
-    return rows + 8 * edges_trials
+    from flybrain_amd import _lib as L
+
+    n, bs = sim.n, sim.bstride
+    tile = sim.pull_tile
+    tpn = min(bs, tile)
+    parts = bs // tpn
+    wpt = tpn // 64
+    words = bs // 64
+    t0 = sim.t
+    hmask = 0
+    for u in range(t0 - 20, t0 + W - 20):
+        if u >= 0:
+            hmask |= 1 << (u % L.REC_SLOTS)
+    act = sim.act.cpu().numpy().reshape(n, parts).view(np.uint64)
+    pairs_src = popcount64(act & np.uint64(hmask)).sum(axis=1)
+    pairs = int((conn.degree().astype(np.int64) * pairs_src).sum())
+    # run the window, then count the neuron-trials that sent in it
+    sim.pull(W)
+    rec = sim.rec.cpu().numpy().reshape(n, L.REC_SLOTS, words).view(np.uint64)
+    slots = [u % L.REC_SLOTS for u in range(t0, t0 + W)]
+    sent = np.bitwise_or.reduce(rec[:, slots, :], axis=1)
+    spiking = int(popcount64(sent).sum())
+    E = conn.n_edges
+    parts_b = {
+        "state": 23 * n * bs,
+        "in_edges": parts * (14 * E + 4 * n),
+        "records_read": 8 * wpt * pairs,
+        "records_written": 8 * W * words * n + 16 * n * parts,
+        "counts": 8 * spiking,
+    }
+    return sum(parts_b.values()), parts_b, {"active_pairs": pairs,
+                                            "spiking_neuron_trials": spiking}
 
 
-def serve_bench(net, conn, args):
+def push_window_bytes(sim, conn, W):
+    """Bytes one push-road window (update + scatter, bstride < 64) must
+    move: state 23 B per neuron-trial as on the pull road, the int32 ring
+    slots the window reads (4 B per neuron-trial and step), and the
+    scatter -- per spiking (neuron, window) row 8 B offsets + 6 B per
+    edge, plus an 8 B int32 read-modify-write per edge, spiking trial and
+    step it spiked at."""
+    import numpy as np
+
+    n, bs = sim.n, sim.bstride
+    before = sim.counts.view(n, bs).cpu().numpy().copy()
+    sim.step_launch(W)
+    after = sim.counts.view(n, bs).cpu().numpy()
+    spikes = (after.astype(np.int64) - before.astype(np.int64))[:, :sim.batch]
+    deg = conn.degree().astype(np.int64)
+    rows = spikes.sum(axis=1) > 0
+    parts_b = {
+        "state": 23 * n * bs,
+        "ring_read": 4 * W * n * bs,
+        "scatter_rows": int((8 + 6 * deg[rows]).sum()),
+        "scatter_adds": int(8 * (deg[:, None] * spikes).sum()),
+    }
+    return sum(parts_b.values()), parts_b, {"spikes": int(spikes.sum())}
+
+
+def window_model(runner, conn, windows=5):
+    """Average bytes per window of the dominant launch, over ``windows``
+    steady-state windows run here (eagerly, after the timed ones)."""
+    s = runner.sim
+    tot, parts_sum, extra_sum = 0, {}, {}
+    for _ in range(windows):
+        if runner.table is not None and not runner.fused:
+            s.drive_table(runner.table, runner.window)
+        if s.road == "pull":
+            b, parts, extra = pull_window_bytes(s, conn, runner.window)
+        else:
+            b, parts, extra = push_window_bytes(s, conn, runner.window)
+        tot += b
+        for k, v in parts.items():
+            parts_sum[k] = parts_sum.get(k, 0) + v
+        for k, v in extra.items():
+            extra_sum[k] = extra_sum.get(k, 0) + v
+    return (tot // windows, {k: v // windows for k, v in parts_sum.items()},
+            {k: v / windows for k, v in extra_sum.items()})
+
+
+def scope_table_bytes(conn, n, B, W, spikes_per_trial_step):
+    """The scope table's per-step formula for a window of W steps:
+    N*B*26 + rows + edge atomics, with the window's average spikes spread
+    over average-degree rows -- the figure behind ``effective_gbs``."""
+    deg = float(conn.degree().mean())
+    spikes = spikes_per_trial_step * B * W
+    return int(BETA * n * B * W + spikes * (8 + 6 * deg) + 8 * spikes * deg)
+
+
+def serve_bench(net, conn, args, rate, count):
     """Serving workload: Poisson arrivals of simulation requests
     (500-20,000 steps, 20-200 driven neurons at 50-250 Hz, 0-50 silenced,
     own seed) served by continuous batching in resident slots, and the same
@@ -120,15 +243,14 @@ def serve_bench(net, conn, args):
 
     from flybrain_amd.serve import Server, SimRequest, poisson_requests
 
-    reqs = poisson_requests(conn.n, args.serve_requests, args.serve_rate,
-                            seed=1)
+    reqs = poisson_requests(conn.n, count, rate, seed=1)
 
     def copies():
         return [SimRequest(**vars(r)) for r in reqs]
 
     warm = [SimRequest(rid=-1 - k, steps=200, drive=reqs[k].drive,
                        rate_hz=reqs[k].rate_hz, seed=k) for k in range(2)]
-    out = {"workload": "poisson_requests", "rate_req_s": args.serve_rate,
+    out = {"workload": "poisson_requests", "rate_req_s": rate,
            "requests": len(reqs),
            "mean_steps": float(np.mean([r.steps for r in reqs]))}
     runs = {}
@@ -143,7 +265,7 @@ def serve_bench(net, conn, args):
         st["road"] = srv.road
         st["shard"] = srv.shard
         out[name] = st
-        log("serve %s: %s" % (name, st))
+        log("serve %s @ %g req/s: %s" % (name, rate, st))
         del srv
     a, b = runs["batched"], runs["b1_back_to_back"]
     out["results_identical"] = bool(
@@ -151,18 +273,16 @@ def serve_bench(net, conn, args):
     return out
 
 
-def profiled_traffic(kernel):
-    """HBM bytes per launch of ``kernel`` at its largest grid, from the
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summaries committed in profiles/
-    (KiB per dispatch, scaled by what the same counters read for a plain
-    256 MiB copy: FETCH x 2.0, WRITE x 1.0).  None when not profiled."""
+def _profile_rows(pattern, kernel, counters):
+    """Mean value per dispatch of ``counters`` for ``kernel`` at its largest
+    grid, from the rocprofv3 summaries committed under profiles/."""
     import glob
 
     best = {}
-    for path in sorted(glob.glob(str(ROOT / "profiles" / "*_hbm_counters.md"))):
+    for path in sorted(glob.glob(str(ROOT / "profiles" / pattern))):
         for line in open(path):
             parts = line.split()
-            if len(parts) < 5 or parts[-3] not in ("FETCH_SIZE", "WRITE_SIZE"):
+            if len(parts) < 5 or parts[-3] not in counters:
                 continue
             head = line[:60]
             if (kernel + "(" not in head and kernel + "<" not in head
@@ -171,14 +291,41 @@ def profiled_traffic(kernel):
             grid, name, val = int(parts[-4]), parts[-3], float(parts[-1])
             if grid >= best.get(name, (0, 0))[0]:
                 best[name] = (grid, val)
-    if "FETCH_SIZE" not in best or "WRITE_SIZE" not in best:
+    return {k: v for k, (_, v) in best.items()}
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of ``kernel`` from the FETCH_SIZE / WRITE_SIZE
+    summaries (KiB per dispatch, scaled by what the same counters read for
+    a plain 256 MiB copy: FETCH x 2.0, WRITE x 1.0).  None when not
+    profiled."""
+    got = _profile_rows("*_hbm_counters.md", kernel,
+                        ("FETCH_SIZE", "WRITE_SIZE"))
+    if "FETCH_SIZE" not in got or "WRITE_SIZE" not in got:
         return None
-    return int(1024 * (2.0 * best["FETCH_SIZE"][1]
-                       + 1.0 * best["WRITE_SIZE"][1]))
+    return int(1024 * (2.0 * got["FETCH_SIZE"] + 1.0 * got["WRITE_SIZE"]))
 
 
-def cpu_baseline(conn, wl, seconds):
-    """The oracle's torch restatement of the reference step on this host."""
+def profiled_valu(kernel):
+    """SQ_INSTS_VALU per launch of ``kernel`` from the committed SQ
+    counter summaries, or None."""
+    return _profile_rows("*_sq_counters.md", kernel,
+                         ("SQ_INSTS_VALU",)).get("SQ_INSTS_VALU")
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_leg(conn, wl, seconds, max_steps=None):
+    """The oracle's torch restatement of the reference step on this host,
+    B = 1, driven like ``wl`` (Bernoulli outcomes drawn on the host)."""
     import numpy as np
     import torch
 
@@ -194,28 +341,47 @@ def cpu_baseline(conn, wl, seconds):
         eng.step(bits)
     steps = 0
     t0 = time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
+    while (time.perf_counter() - t0 < seconds
+           and (max_steps is None or steps < max_steps)):
         bits[:] = 0
         kick = wl.drive_neuron[rng.random(wl.drive_neuron.size) < p]
         bits[0, kick] = 1
         eng.step(bits)
         steps += 1
     wall = time.perf_counter() - t0
+    return steps, wall, cores
+
+
+def cpu_baseline(conn, seconds):
+    """Configs 2 (138,639 neurons; a bounded sample of the B = 1 workload)
+    and 1 (10,000 neurons, 1,000 steps) on the host's cores."""
+    from flybrain_amd import connectome as C
+    from flybrain_amd import workloads as WL
+
+    steps, wall, cores = cpu_leg(conn, WL.plain_b1(conn.n, seed=0), seconds)
+    small = C.small()
+    s1, w1, _ = cpu_leg(small, WL.plain_b1(small.n, seed=0), 120.0,
+                        max_steps=1000)
     return {"value": round(steps * 1e-4 / wall, 5), "unit": "trial-s/s",
             "cores": cores, "kind": "port",
             "sample": "%d steps of the B=1 workload (138,639 neurons, 20 "
                       "neurons at 150 Hz), torch CPU csrmv restatement of "
                       "the reference step, %.1f s" % (steps, wall),
-            "ms_per_step": round(1e3 * wall / steps, 3)}
+            "ms_per_step": round(1e3 * wall / steps, 3),
+            "cpu_model": cpu_model(),
+            "config1": {"workload": "10,000 neurons, %d edges, 20 neurons "
+                                    "at 150 Hz, B=1" % small.n_edges,
+                        "steps": s1, "wall_s": round(w1, 3),
+                        "value": round(s1 * 1e-4 / w1, 5),
+                        "ms_per_step": round(1e3 * w1 / s1, 4)}}
 
 
 def summarize(paths):
     """Per-kernel summary of rocprofv3 databases (kernel trace or --pmc).
 
     Dispatches are grouped by kernel name and grid size (which tells the
-    B = 256 and B = 1 workloads apart).  For counter databases the mean
-    value per dispatch is printed next to what the same counter read for
-    the 256 MiB calibration copy (``--calib-copy``).
+    workloads apart).  For counter databases the mean value per dispatch
+    is printed.
     """
     import sqlite3
 
@@ -227,8 +393,6 @@ def summarize(paths):
             "max(duration), sum(duration) from kernels group by name, grid_x "
             "order by sum(duration) desc").fetchall()
         if rows and rows[0][2]:
-            # the median too: a run's quieter first windows (state from
-            # rest) pull the mean below the steady-state duration
             print("%-60s %10s %7s %12s %12s %12s %12s" % (
                 "kernel", "grid", "calls", "avg_us", "median_us", "min_us",
                 "max_us"))
@@ -260,25 +424,86 @@ def _short(name):
     return name[:60]
 
 
+def run_leg(name, net, conn, args, wl, steps):
+    """Preroll, warm-up and ``steps`` timed steps of ``wl``; kernel times
+    from HIP events afterwards."""
+    from flybrain_amd.batch import BatchRunner
+
+    tuned = None
+    if args.road == "tuned":
+        from flybrain_amd.autotune import tune
+        t1 = time.perf_counter()
+        tuned = tune(net, wl)
+        log("%s: tuned %s in %.1f s, ms per window %s"
+            % (name, tuned["best"], time.perf_counter() - t1,
+               tuned["times"]))
+    r = BatchRunner(net, wl, counts=True, road=args.road)
+    W = r.window
+    # a graph that fits the timed steps, so that they replay it
+    gw = min(args.graph_windows, steps // W) if not args.no_graph else 0
+    if gw > 0:
+        r.capture(gw)
+    use = gw > 0
+    sync = __import__("torch").cuda.synchronize
+    r.advance(args.preroll, use_graph=use)
+    r.advance(args.warmup, use_graph=use)
+    c0 = r.counts().sum()
+    wall = timed(lambda: r.advance(steps, use_graph=use), sync)
+    spikes = int(r.counts().sum() - c0)
+    kt = kernel_times(r, 20)
+    res = dict(wl=wl, runner=r, wall=wall, steps=steps, spikes=spikes,
+               kt=kt, tuned=tuned, graph_windows=gw,
+               tps=wl.batch * steps * 1e-4 / wall)
+    log("%s: B=%d %d steps %.4f s  %.3f us/step  %.2f trial-s/s  "
+        "spikes/trial-step %.2f  kernels(ms) %s  graph %d windows"
+        % (name, wl.batch, steps, wall, 1e6 * wall / steps, res["tps"],
+           spikes / (wl.batch * steps),
+           {k: round(v, 5) for k, v in kt.items()}, gw))
+    return res
+
+
+def leg_summary(rr):
+    r = rr["runner"]
+    return {"workload": rr["wl"].name, "batch": rr["wl"].batch,
+            "value": round(rr["tps"], 3), "steps": rr["steps"],
+            "ms_per_step": round(1e3 * rr["wall"] / rr["steps"], 6),
+            "graph_windows": rr["graph_windows"],
+            "road": r.sim.road,
+            "pull_tile": r.sim.pull_tile if r.sim.road == "pull" else None,
+            "autotuned": rr["tuned"]["times"] if rr["tuned"] else None,
+            "spikes_per_trial_step": round(
+                rr["spikes"] / (rr["wl"].batch * rr["steps"]), 3),
+            "kernels_ms": {k: round(v, 5) for k, v in rr["kt"].items()}}
+
+
 def main():
     if len(sys.argv) > 2 and sys.argv[1] == "--summarize":
         summarize(sys.argv[2:])
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--preroll", type=int, default=1000,
+                    help="untimed steps from rest before the warm-up of "
+                         "every leg (working activity)")
     ap.add_argument("--b1-steps", type=int, default=10_000)
     ap.add_argument("--b64-steps", type=int, default=10_000)
+    ap.add_argument("--plain-steps", type=int, default=2_000,
+                    help="timed steps of the plain_b256 leg")
     ap.add_argument("--graph-windows", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=("all", "b256", "b1", "b64"),
+    ap.add_argument("--only", choices=("all", "b256", "b1", "b64",
+                                       "plain_b256"),
                     default="all")
     ap.add_argument("--serve-requests", type=int, default=128,
-                    help="requests of the serving run (0: skip it)")
+                    help="requests of the 200 req/s serving run (0: skip "
+                         "serving)")
     ap.add_argument("--serve-rate", type=float, default=200.0,
                     help="Poisson arrival rate of requests per second")
+    ap.add_argument("--serve-low-rate", type=float, default=20.0)
+    ap.add_argument("--serve-low-requests", type=int, default=48)
     ap.add_argument("--serve-slots", type=int, default=256)
     ap.add_argument("--serve-shard", type=int, default=16,
                     help="slots per shard (one DeviceSim each)")
@@ -290,6 +515,8 @@ def main():
                          "push below")
     ap.add_argument("--no-graph", action="store_true",
                     help="eager launches (for per-dispatch counter runs)")
+    ap.add_argument("--no-model", action="store_true",
+                    help="skip the per-window byte model (profiling runs)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 256 MiB elementwise copy (counter "
                          "calibration: known bytes read and written)")
@@ -301,15 +528,12 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
     from flybrain_amd import connectome as C
     from flybrain_amd import workloads as WL
-    from flybrain_amd.batch import BatchRunner
     from flybrain_amd.device import DeviceNet
 
-    sync = torch.cuda.synchronize
     t0 = time.perf_counter()
     conn = C.full()
     log("connectome %d neurons, %d edges, %s (%.1f s)"
@@ -321,136 +545,114 @@ def main():
            "scaling": "weak", "vs_baseline": None, "dtype": "f32+i32",
            "data": "synthetic"}
 
+    total = lambda k: args.preroll + args.warmup + k  # noqa: E731
+    makers = {
+        "b256": (lambda: WL.sensory_b256(conn.n, seed=0,
+                                         steps=total(args.steps)),
+                 args.steps),
+        "b1": (lambda: WL.plain_b1(conn.n, seed=0, steps=total(args.b1_steps)),
+               args.b1_steps),
+        "plain_b256": (lambda: WL.plain_replicated(
+            conn.n, seed=0, batch=256, steps=total(args.plain_steps)),
+            args.plain_steps),
+        "b64": (lambda: WL.trials_b64(conn.n, seed=0,
+                                      steps=total(args.b64_steps)),
+                args.b64_steps),
+    }
+    names = (("b256", "b1", "plain_b256", "b64") if args.only == "all"
+             else (args.only,))
     res = {}
-    for name in (("b256", "b1", "b64") if args.only == "all"
-                 else (args.only,)):
-        if name == "b256":
-            wl = WL.sensory_b256(conn.n, seed=0, steps=args.warmup + args.steps)
-            steps = args.steps
-        elif name == "b64":  # per-trial drive, autotuned trial-block width
-            wl = WL.trials_b64(conn.n, seed=0,
-                               steps=args.warmup + args.b64_steps)
-            steps = args.b64_steps
-        else:
-            wl = WL.plain_b1(conn.n, seed=0, steps=args.warmup + args.b1_steps)
-            steps = args.b1_steps
-        tuned = None
-        if args.road == "tuned":
-            from flybrain_amd.autotune import tune
-            t1 = time.perf_counter()
-            tuned = tune(net, wl)
-            log("%s: tuned %s in %.1f s, ms per window %s"
-                % (name, tuned["best"], time.perf_counter() - t1,
-                   tuned["times"]))
-        r = BatchRunner(net, wl, counts=True, road=args.road)
-        if not args.no_graph:
-            r.capture(args.graph_windows)
-        r.advance(args.warmup, use_graph=not args.no_graph)
-        wall = timed(lambda: r.advance(steps, use_graph=not args.no_graph),
-                     sync)
-        spikes = int(r.counts().sum())
-        kt = kernel_times(r, 20)
-        res[name] = dict(wl=wl, runner=r, wall=wall, steps=steps,
-                         spikes=spikes, kt=kt, tuned=tuned,
-                         tps=wl.batch * steps * 1e-4 / wall)
-        log("%s: B=%d %d steps %.4f s  %.3f us/step  %.2f trial-s/s  "
-            "spikes/trial-step %.2f  kernels(ms) %s"
-            % (name, wl.batch, steps, wall, 1e6 * wall / steps,
-               res[name]["tps"],
-               spikes / (wl.batch * (steps + args.warmup)),
-               {k: round(v, 5) for k, v in kt.items()}))
+    for name in names:
+        mk, steps = makers[name]
+        res[name] = run_leg(name, net, conn, args, mk(), steps)
 
     if args.calib_copy:
         a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
         b = a + 1.0  # reads 256 MiB, writes 256 MiB
-        sync()
+        torch.cuda.synchronize()
         log("calibration copy: 256 MiB read + 256 MiB written (%s)"
             % float(b[0]))
         del a, b
-    head = res.get("b256") or res.get("b1") or res.get("b64")
+
+    head = res.get("b256") or res[names[0]]
     wl = head["wl"]
-    n, B, W = conn.n, wl.batch, head["runner"].window
+    r = head["runner"]
+    n, B, W = conn.n, wl.batch, r.window
     kt = head["kt"]
     dom = max((k for k in kt if k != "drive"), key=lambda k: kt[k])
-    alg = 0
-    if dom in ("update", "pull", "update_scatter"):
-        alg += BETA * n * B * W
-    if dom in ("scatter", "pull", "update_scatter"):
-        # spikes of the last window, from a short raster run
-        from flybrain_amd.batch import BatchRunner as _BR
-        rr = _BR(net, wl, counts=False, raster_len=W)
-        rr.advance(min(wl.steps, args.warmup + head["steps"]) // W * W - W)
-        rr.advance(W, use_graph=False)
-        rows = rr.sim.read_raster(rr.t - W, rr.t)
-        alg += scatter_bytes(conn, rows)
-        del rr
-    achieved = alg / (kt[dom] * 1e-3) / 1e9
+    hs = r.sim
     kname = {"update": "k_update", "update_scatter": "k_update",
-             "pull": "k_pull"}.get(dom) or (
-        "k_scatter_steps" if head["runner"].sim.bstride >= 64 else "k_scatter")
-    if dom == "pull":  # the instantiation that ran: window, stride, tile
-        hs = head["runner"].sim
-        kname = "k_pull<%d, %d, %d>" % (20, hs.bstride, hs.pull_tile)
+             "pull": "k_pull<%d, %d, %d>" % (W, hs.bstride, hs.pull_tile),
+             "scatter": "k_scatter_steps"}[dom]
+    spts = head["spikes"] / (B * head["steps"])
+    roof = {"bound": "hbm", "kernel": kname, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "avg_launch_ms": round(kt[dom], 5)}
+    if not args.no_model and dom in ("pull", "update_scatter"):
+        wb, parts, extra = window_model(r, conn)
+        achieved = wb / (kt[dom] * 1e-3) / 1e9
+        roof.update({"achieved": round(achieved, 1),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "window_bytes": int(wb), "window_bytes_parts": parts,
+                     "window_model_counts": extra})
     traffic = profiled_traffic(kname)
+    roof["traffic"] = traffic
+    roof["traffic_gbs"] = (round(traffic / (kt[dom] * 1e-3) / 1e9, 1)
+                           if traffic else None)
+    valu = profiled_valu(kname)
+    roof["valu_insts_per_launch"] = valu
+    roof["valu_issue_frac"] = (round(valu / (VALU_ISSUE_PER_S
+                                             * kt[dom] * 1e-3), 4)
+                               if valu else None)
+    st_b = scope_table_bytes(conn, n, B, W, spts)
+    roof["effective_gbs"] = round(st_b / (kt[dom] * 1e-3) / 1e9, 1)
+    roof["effective_bytes_note"] = (
+        "scope-table formula, 26 B per neuron-trial-step priced every step "
+        "plus the scatter term; not bytes the windowed kernel moves")
+    floor_ms = n * B / 64 * PLAIN_STEP_VALU * W / VALU_ISSUE_PER_S * 1e3
     out.update({
         "value": round(head["tps"], 3),
         "steps": head["steps"], "warmup": args.warmup,
         "ms_per_step": round(1e3 * head["wall"] / head["steps"], 6),
         "config": {"workload": wl.name, "neurons": n,
                    "edges": conn.n_edges, "batch": B,
-                   "road": head["runner"].sim.road,
-                   "pull_tile": (head["runner"].sim.pull_tile
-                                 if head["runner"].sim.road == "pull"
-                                 else None),
+                   "road": hs.road,
+                   "pull_tile": hs.pull_tile if hs.road == "pull" else None,
                    "autotuned": (head["tuned"]["times"] if head["tuned"]
                                  else None),
-                   "window_steps": W, "graph_windows": args.graph_windows,
+                   "window_steps": W,
+                   "graph_windows": head["graph_windows"],
+                   "preroll_steps": args.preroll,
                    "connectome": conn.fingerprint()},
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic,
-                     "traffic_gbs": (round(traffic / (kt[dom] * 1e-3) / 1e9, 1)
-                                     if traffic else None),
-                     "algorithmic_bytes_per_launch": int(alg),
-                     "avg_launch_ms": round(kt[dom], 5)},
+        "roofline": roof,
         "kernels_ms": {k: round(v, 5) for k, v in kt.items()},
+        "kernel_ms_per_step": round(sum(kt.values()) / W, 6),
+        "recurrence_floor_ms_per_window": round(floor_ms, 5),
     })
-    if "b1" in res and "b256" in res:
-        out["b1"] = {"value": round(res["b1"]["tps"], 3),
-                     "steps": res["b1"]["steps"],
-                     "ms_per_step": round(1e3 * res["b1"]["wall"]
-                                          / res["b1"]["steps"], 6),
-                     "kernels_ms": {k: round(v, 5)
-                                    for k, v in res["b1"]["kt"].items()}}
-        out["ratio_b256_over_b1"] = round(res["b256"]["tps"]
-                                          / res["b1"]["tps"], 3)
-    if "b64" in res:  # 64 trials, each with its own 20-200 driven neurons
-        r64 = res["b64"]
-        out["b64"] = {"workload": r64["wl"].name,
-                      "value": round(r64["tps"], 3), "steps": r64["steps"],
-                      "ms_per_step": round(1e3 * r64["wall"]
-                                           / r64["steps"], 6),
-                      "road": r64["runner"].sim.road,
-                      "pull_tile": (r64["runner"].sim.pull_tile
-                                    if r64["runner"].sim.road == "pull"
-                                    else None),
-                      "autotuned": (r64["tuned"]["times"] if r64["tuned"]
-                                    else None),
-                      "kernels_ms": {k: round(v, 5)
-                                     for k, v in r64["kt"].items()}}
     for name, rr in res.items():
-        out.setdefault("spikes_per_trial_step", {})[name] = round(
-            rr["spikes"] / (rr["wl"].batch * (rr["steps"] + args.warmup)),
-            3)
+        if rr is not head:
+            out[name] = leg_summary(rr)
+    out["spikes_per_trial_step"] = {
+        name: round(rr["spikes"] / (rr["wl"].batch * rr["steps"]), 3)
+        for name, rr in res.items()}
+    if "b1" in res:
+        b1 = res["b1"]["tps"]
+        if "b256" in res:
+            out["ratio_b256_over_b1"] = round(res["b256"]["tps"] / b1, 3)
+        if "plain_b256" in res:
+            out["ratio_plain_b256_over_b1"] = round(
+                res["plain_b256"]["tps"] / b1, 3)
     if args.serve_requests > 0 and args.only == "all":
-        sv = serve_bench(net, conn, args)
+        sv = serve_bench(net, conn, args, args.serve_rate,
+                         args.serve_requests)
         out["serving"] = sv
         out["p50_ms"] = sv["batched"]["p50_ms"]
         out["p99_ms"] = sv["batched"]["p99_ms"]
+        if args.serve_low_requests > 0:
+            out["serving_low_rate"] = serve_bench(
+                net, conn, args, args.serve_low_rate,
+                args.serve_low_requests)
     if not args.no_cpu:
-        b1 = WL.plain_b1(conn.n, seed=0)
-        out["cpu_baseline"] = cpu_baseline(conn, b1, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(conn, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

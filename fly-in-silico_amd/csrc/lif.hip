@@ -510,6 +510,39 @@ __device__ __forceinline__ uint32_t run_window_plain(const fis_sim& s, Cell& c,
   return fired;
 }
 
+// Whether a neuron-trial of a plain wave (no flag, no kick, a full window)
+// sits at a fixed point of the step for the whole window: no synaptic
+// input at any of its WMAX steps, no spike at the previous step, g = +0.0
+// and v unchanged by the leak step v + a_mem * (g - (v - v_rest)) without
+// reaching threshold.  Then every step of the window maps (v, g) to
+// itself: the gated input w_scale * 0 = +0.0 leaves g_new = g * a_syn + 0
+// at +0.0 whether the gate is open or not, and the membrane term is the
+// one just evaluated; only the refractory counter moves, by one per step.
+// Exactly the values run_window_plain would compute, without the chain.
+template <int WMAX>
+__device__ __forceinline__ bool quiet_cell(const fis_sim& s, const Cell& c,
+                                           const int32_t (&in)[WMAX]) {
+  int32_t any = 0;
+#pragma unroll
+  for (int k = 0; k < WMAX; ++k) any |= in[k];
+  const float t1 = c.v - s.v_rest;
+  const float t2 = c.g - t1;
+  const float t3 = s.a_mem * t2;
+  const float vv = c.v + t3;
+  return any == 0 && !c.sp && __float_as_uint(c.g) == 0u &&
+         __float_as_uint(vv) == __float_as_uint(c.v) && !(vv > s.v_th);
+}
+
+// The store of a neuron-trial that stayed at its fixed point for a window:
+// v and g are unchanged, only the refractory counter moved (no spike, so
+// no count, and the last step's spike output is 0).
+__device__ __forceinline__ void store_quiet(const fis_sim& s, const Cell& c,
+                                            int idx, uint32_t st_old) {
+  const uint32_t st = (uint32_t)min(c.r, (int)FIS_ST_RMAX);
+  if (st != st_old) s.st[idx] = (uint8_t)st;
+  if (s.spikes_out) s.spikes_out[idx] = 0.0f;
+}
+
 // A full window of lif_step for a wave with no fire mask off and no hold
 // (the push road's common case, also with driven neurons): as
 // run_window_plain, with the Poisson kick term kept and the gate threshold
@@ -696,10 +729,17 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
   const bool kick_path =
       W == WMAX &&
       !any_lane(act && (fl & (FIS_FIRE_OFF | FIS_HOLD)) != 0u);
-  const uint32_t fired_bits =
-      !act ? 0u
-           : (kick_path ? run_window_kick<WMAX>(s, c, in, sb, stim)
-                        : run_window<WMAX>(s, c, in, sb, stim, W));
+  // a wave whose every neuron-trial sits at a fixed point of the step with
+  // no kick skips the recurrence (quiet_cell; the gate threshold and the
+  // transmit flag do not matter without input or spikes)
+  const bool quiet =
+      kick_path && !any_lane(act && !(sb == 0u && quiet_cell(s, c, in)));
+  uint32_t fired_bits = 0u;
+  if (quiet)
+    c.r += WMAX;
+  else if (act)
+    fired_bits = kick_path ? run_window_kick<WMAX>(s, c, in, sb, stim)
+                           : run_window<WMAX>(s, c, in, sb, stim, W);
   const int64_t left = t_stop - t0;  // steps whose spikes still count
   const int kstop = left <= 0 ? 0 : (left >= W ? W : (int)left);
   const uint32_t nspk = __popc(fired_bits & ((1u << kstop) - 1u));
@@ -707,7 +747,12 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
   if constexpr (FUSED) {
     // below 64 trials the wave adds its own spikes into the ring slots no
     // step of this window reads (no event records, no scatter launch)
-    if (act) store_cell(s, c, idx, pre, nspk);
+    if (act) {
+      if (quiet)
+        store_quiet(s, c, idx, st);
+      else
+        store_cell(s, c, idx, pre, nspk);
+    }
     const uint32_t sent = transmit ? fired_bits : 0u;
     const bool any_sent = any_lane(sent != 0u);
     if (s.raster || any_sent) {
@@ -758,7 +803,12 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
     }
   }
 
-  if (act) store_cell(s, c, idx, pre, nspk);
+  if (act) {
+    if (quiet)
+      store_quiet(s, c, idx, st);
+    else
+      store_cell(s, c, idx, pre, nspk);
+  }
   if (bs < kWave && lane == 0 && act)
     s.ev_count[(int64_t)blockIdx.x * FIS_LINE_INTS + (wave & 3)] = nev;
 
@@ -1031,7 +1081,6 @@ void k_pull(fis_sim s, int W,
     st = s.st[idx];
     fl = s.fl[idx];
     sb = s.sbits[idx];
-    if (s.vhold) vh = s.vhold[idx];  // used only where the flag says HOLD
   }
 
   {  // 1. gather.  The tile's in-edges are read 64 at a time, one per lane,
@@ -1135,6 +1184,9 @@ void k_pull(fis_sim s, int W,
 
   // 2. update
   if (sb) s.sbits[idx] = 0u;  // the window's kicks are consumed
+  // the clamp value is read only where the flag says HOLD (by now fl has
+  // long arrived; 4 B per neuron-trial and window not read otherwise)
+  if (act && (fl & FIS_HOLD) && s.vhold) vh = s.vhold[idx];
   Cell c = make_cell(s, fl, vh, st, v, g);
   const Stim stim = make_stim(s);
   const bool transmit = act && !(fl & FIS_OUT_OFF);
@@ -1164,14 +1216,26 @@ void k_pull(fis_sim s, int W,
   }
 
   // bit k: this lane fired at step k.  Waves with no flag, no kick and no
-  // trial end in a full window (most of them) take the plain recurrence.
+  // trial end in a full window (most of them) take the plain recurrence;
+  // of those, waves whose every lane sits at a fixed point of the step
+  // skip it (quiet_cell).
   const bool plain =
       W == WMAX && !any_lane(!act || fl != 0u || sb != 0u || kstop != W);
-  const uint32_t fired_bits =
-      plain ? run_window_plain<WMAX>(s, c, in)
-            : (act ? run_window<WMAX>(s, c, in, sb, stim, W) : 0u);
+  const bool quiet = plain && !any_lane(!quiet_cell(s, c, in));
+  uint32_t fired_bits = 0u;
+  if (quiet)
+    c.r += WMAX;  // the counter's WMAX increments; v, g and sp stay
+  else if (plain)
+    fired_bits = run_window_plain<WMAX>(s, c, in);
+  else if (act)
+    fired_bits = run_window<WMAX>(s, c, in, sb, stim, W);
   const uint32_t nspk = __popc(fired_bits & ((1u << kstop) - 1u));
-  if (act) store_cell(s, c, idx, pre, nspk);
+  if (act) {
+    if (quiet)
+      store_quiet(s, c, idx, st);
+    else
+      store_cell(s, c, idx, pre, nspk);
+  }
 
   // The wave's record words of the window (and raster rows): lane k < W
   // owns step k's.  This wave's 64 x WMAX cells are its own and already
@@ -1435,7 +1499,7 @@ extern "C" {
 
 int fis_abi_version(void) { return FIS_ABI_VERSION; }
 
-int fis_ev_spmv_abi(void) { return 3; }
+int ev_spmv_abi(void) { return 3; }
 
 int64_t fis_struct_layout(int what) {
   switch (what) {
@@ -1465,24 +1529,24 @@ int64_t fis_n_waves(int32_t n, int32_t bstride) {
   return (nb + kWave - 1) / kWave;
 }
 
-int fis_ev_spmv_i32(const int32_t* crow, const int32_t* col, const float* val,
-                    const float* sp, const float* mo, float* out, int n,
-                    int batch, int threads, void* stream) {
+int ev_spmv_i32(const int* crow, const int* col, const float* val,
+                const float* sp, const float* mo, float* out, int n, int batch,
+                int threads, void* stream) {
   (void)threads;
   if (n <= 0 || batch <= 0) return (int)hipErrorInvalidValue;
   dim3 grid(grid_for(n), (unsigned)batch);
-  hipLaunchKernelGGL(k_ev_spmv<int32_t>, grid, dim3(kBlock), 0,
+  hipLaunchKernelGGL(k_ev_spmv<int>, grid, dim3(kBlock), 0,
                      (hipStream_t)stream, crow, col, val, sp, mo, out, n);
   return (int)hipGetLastError();
 }
 
-int fis_ev_spmv_i64(const int64_t* crow, const int64_t* col, const float* val,
-                    const float* sp, const float* mo, float* out, int n,
-                    int batch, int threads, void* stream) {
+int ev_spmv_i64(const long long* crow, const long long* col,
+                const float* val, const float* sp, const float* mo, float* out,
+                int n, int batch, int threads, void* stream) {
   (void)threads;
   if (n <= 0 || batch <= 0) return (int)hipErrorInvalidValue;
   dim3 grid(grid_for(n), (unsigned)batch);
-  hipLaunchKernelGGL(k_ev_spmv<int64_t>, grid, dim3(kBlock), 0,
+  hipLaunchKernelGGL(k_ev_spmv<long long>, grid, dim3(kBlock), 0,
                      (hipStream_t)stream, crow, col, val, sp, mo, out, n);
   return (int)hipGetLastError();
 }

@@ -135,12 +135,12 @@ def lib():
     sims = ctypes.POINTER(FisSim)
     sig = {
         "fis_abi_version": ([], I),
-        "fis_ev_spmv_abi": ([], I),
+        "ev_spmv_abi": ([], I),
         "fis_struct_layout": ([I], I64),
         "fis_n_waves": ([ctypes.c_int32, ctypes.c_int32], I64),
         "fis_n_blocks": ([ctypes.c_int32, ctypes.c_int32], I64),
-        "fis_ev_spmv_i32": ([P] * 6 + [I, I, I, P], I),
-        "fis_ev_spmv_i64": ([P] * 6 + [I, I, I, P], I),
+        "ev_spmv_i32": ([P] * 6 + [I, I, I, P], I),
+        "ev_spmv_i64": ([P] * 6 + [I, I, I, P], I),
         "fis_drive_rates": ([sims, P, I, P], I),
         "fis_drive_table": ([sims, P, I, P, I, I, I, I, P], I),
         "fis_drive_tape": ([sims, P, P, I64, I, P], I),
@@ -192,5 +192,5 @@ def exported_symbols():
     import re
 
     text = HEADER.read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|int64_t)\s+(fis_\w+)\s*\(", text,
-                                 re.M)))
+    return sorted(set(re.findall(
+        r"^\s*(?:int|int64_t)\s+((?:fis|ev_spmv)_\w+)\s*\(", text, re.M)))

@@ -9,8 +9,10 @@ depends on the batch, the connectome and how active it is, so
 warm-up to reach its working activity, then a few eager windows bracketed
 by device events -- and keeps the fastest.  Results are cached in a JSON
 file (``$FLYBRAIN_TUNE_CACHE``, default ``~/.cache/flybrain_amd/tune.json``)
-keyed by device, connectome fingerprint, workload and batch; the cache is
-only an optimisation (an unwritable location is skipped).
+keyed by device, the library build (ABI version and a hash of libfis.so),
+connectome fingerprint, workload name, batch and a hash of the workload's
+arrays; the cache is only an optimisation (an unwritable location is
+skipped).
 """
 
 from __future__ import annotations
@@ -36,11 +38,43 @@ def candidates(batch):
     return out
 
 
+def library_fingerprint():
+    """ABI version and a hash of the built libfis.so: a choice measured with
+    other kernels is not reused after a rebuild."""
+    import hashlib
+
+    h = hashlib.sha256()
+    try:
+        h.update(L.LIB_PATH.read_bytes())
+    except OSError:
+        h.update(b"missing")
+    return "abi%d-%s" % (L.ABI_VERSION, h.hexdigest()[:16])
+
+
+def workload_fingerprint(wl):
+    """Hash of everything that sets a workload's activity: seeds, drive
+    entries, rate schedule and m_out pairs (not only its name)."""
+    import hashlib
+
+    import numpy as np
+
+    h = hashlib.sha256()
+    h.update(("%s|%d|%d" % (wl.name, wl.batch, wl.seg_steps)).encode())
+    for a in (wl.seeds, wl.drive_neuron, wl.drive_trial, wl.drive_group,
+              wl.rate_hz, wl.out_neuron, wl.out_trial):
+        a = np.ascontiguousarray(a)
+        h.update(str((a.dtype.str, a.shape)).encode())
+        h.update(a.tobytes())
+    return h.hexdigest()[:16]
+
+
 def cache_key(net, wl):
     import torch
     dev = torch.cuda.get_device_name(net.device) if net.device.type == "cuda" \
         else str(net.device)
-    return "%s|%s|%s|%d" % (dev, net.conn.fingerprint(), wl.name, wl.batch)
+    return "%s|%s|%s|%s|%d|%s" % (dev, library_fingerprint(),
+                                  net.conn.fingerprint(), wl.name, wl.batch,
+                                  workload_fingerprint(wl))
 
 
 def _load(path):

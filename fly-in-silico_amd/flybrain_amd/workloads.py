@@ -10,8 +10,9 @@ A ``Workload`` lists, for ``batch`` independent trials,
   * m_out mask pairs (neuron, trial): the neuron's spikes are not
     transmitted in that trial (Engine.set_masks(out=...), sim.py:510-571).
 
-The three workloads on the 138,639-neuron / 15,091,983-edge graph:
+The workloads on the 138,639-neuron / 15,091,983-edge graph:
   plain_b1       20 seeded neurons at 150 Hz, B = 1, 10,000 steps
+  plain_b256     the plain_b1 drive in 256 trials with distinct seeds
   trials_b64     B = 64, per trial 20-200 neurons at U(50, 250) Hz
   sensory_b256   B = 256, four fixed index sets (1,116 + 1,133 olfactory-
                  type, 410 gustatory-type, 10,582 photoreceptor-type) with
@@ -73,6 +74,22 @@ def plain_b1(n, seed=0, steps=10_000, n_driven=20, rate_hz=150.0):
                     np.zeros(idx.size, np.int64), np.zeros(idx.size, np.int64),
                     np.array([[rate_hz]], np.float32), 1 << 30,
                     np.zeros(0, np.int64), np.zeros(0, np.int64))
+
+
+def plain_replicated(n, seed=0, batch=256, steps=10_000, n_driven=20,
+                     rate_hz=150.0):
+    """The plain_b1 drive (the same 20 neurons at 150 Hz) in ``batch``
+    trials, each with its own Philox seed: every trial is the plain
+    workload, so its rate is directly comparable with B = 1."""
+    one = plain_b1(n, seed=seed, steps=steps, n_driven=n_driven,
+                   rate_hz=rate_hz)
+    rng = np.random.Generator(np.random.PCG64(seed + 7_000_001))
+    idx = one.drive_neuron
+    return Workload("plain_b%d" % batch, batch, steps, _seeds(rng, batch),
+                    np.tile(idx, batch), np.repeat(np.arange(batch), idx.size),
+                    np.zeros(idx.size * batch, np.int64), one.rate_hz,
+                    one.seg_steps, np.zeros(0, np.int64),
+                    np.zeros(0, np.int64))
 
 
 def trials_b64(n, seed=0, batch=64, steps=10_000):

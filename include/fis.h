@@ -9,9 +9,10 @@
  *
  * Two groups of entry points:
  *
- *  1. fis_ev_spmv_i32 / fis_ev_spmv_i64 / fis_ev_spmv_abi -- drop-in twins
- *     of the reference's event SpMV C ABI, lab/flybrain/ev_spmv.cu:114-137
- *     (ctypes signatures lab/flybrain/evspmv.py:896-903):
+ *  1. ev_spmv_i32 / ev_spmv_i64 / ev_spmv_abi -- the reference's event SpMV
+ *     C ABI under its own names and signatures, lab/flybrain/ev_spmv.cu:
+ *     114-137, which lab/flybrain/evspmv.py:886-903 binds by name through
+ *     ctypes (so this library loads where the reference's DLL did):
  *         out[b, post] += sum_pre W[post, pre] * sp[b, pre] * mo[b, pre]
  *     over a CSR of W^T (row = presynaptic neuron), float atomics.
  *
@@ -202,21 +203,27 @@ typedef struct {
 
 int fis_abi_version(void);
 
-/* ---- reference-ABI twins (lab/flybrain/ev_spmv.cu:114-137) ---------- */
+/* ---- the reference's event-SpMV ABI (lab/flybrain/ev_spmv.cu:114-137) --
+ * Same names, argument lists and return convention (a hipError_t as int,
+ * 0 = success; the reference returns cudaGetLastError()), bound by
+ * lab/flybrain/evspmv.py:886-903 with argtypes [c_void_p] * 6 + [c_int] * 3
+ * + [c_void_p].  The caller zeroes `out` first (evspmv.py:1131); the kernel
+ * adds into it with float atomics (order-free under the exactness gate of
+ * evspmv.py:957-994).  n <= 0 or batch <= 0 returns hipErrorInvalidValue
+ * without a launch. */
 
-/* ev_spmv_i32 (ev_spmv.cu:114-121): int32 W^T CSR.  `threads` is accepted
- * for signature compatibility; the launch shape is the library's own. */
-int fis_ev_spmv_i32(const int32_t* crow, const int32_t* col, const float* val,
-                    const float* sp, const float* mo, float* out, int n,
-                    int batch, int threads, void* stream);
-/* ev_spmv_i64 (ev_spmv.cu:123-130): int64 W^T CSR. */
-int fis_ev_spmv_i64(const int64_t* crow, const int64_t* col, const float* val,
-                    const float* sp, const float* mo, float* out, int n,
-                    int batch, int threads, void* stream);
-/* ev_spmv_abi (ev_spmv.cu:136-137) returns 3 in the reference; this
- * library's twin returns the same number: the argument list above is that
- * ABI, unchanged. */
-int fis_ev_spmv_abi(void);
+/* replaces ev_spmv_i32 (ev_spmv.cu:114-121): int32 W^T CSR.  `threads` is
+ * accepted for the signature; the launch shape is the library's own. */
+int ev_spmv_i32(const int* crow, const int* col, const float* val,
+                const float* sp, const float* mo, float* out, int n, int batch,
+                int threads, void* stream);
+/* replaces ev_spmv_i64 (ev_spmv.cu:123-130): int64 W^T CSR. */
+int ev_spmv_i64(const long long* crow, const long long* col,
+                const float* val, const float* sp, const float* mo, float* out,
+                int n, int batch, int threads, void* stream);
+/* replaces ev_spmv_abi (ev_spmv.cu:136-137): returns 3, the number
+ * evspmv.py:888-894 checks before binding the two entry points above. */
+int ev_spmv_abi(void);
 
 /* ---- the fused engine ----------------------------------------------- */
 

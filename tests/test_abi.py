@@ -21,6 +21,9 @@ def lib():
 def test_exports_every_declared_symbol(lib):
     declared = L.exported_symbols()
     assert len(declared) >= 12
+    # the reference's event-SpMV entry points, under their own names
+    # (lab/flybrain/ev_spmv.cu:114-137, bound by name at evspmv.py:886-903)
+    assert {"ev_spmv_i32", "ev_spmv_i64", "ev_spmv_abi"} <= set(declared)
     raw = ctypes.CDLL(str(L.LIB_PATH))
     for name in declared:
         assert hasattr(raw, name), name
@@ -32,7 +35,7 @@ def test_exports_every_declared_symbol(lib):
 
 def test_versions_and_layout(lib):
     assert lib.fis_abi_version() == L.ABI_VERSION
-    assert lib.fis_ev_spmv_abi() == 3  # the reference's ev_spmv_abi()
+    assert lib.ev_spmv_abi() == 3  # the reference's ev_spmv_abi()
     assert lib.fis_struct_layout(0) == ctypes.sizeof(L.FisSim)
     assert lib.fis_struct_layout(5) == 16
     assert lib.fis_struct_layout(6) == ctypes.sizeof(L.FisDriveEntry)
@@ -55,8 +58,8 @@ def test_bad_arguments_rejected_before_launch(lib):
     s.n, s.batch, s.bstride = 10, 3, 3  # 3 is not a valid stride
     assert lib.fis_update(ctypes.byref(s), 1, None) != 0
     assert lib.fis_scatter(ctypes.byref(s), 1, None) != 0
-    assert lib.fis_ev_spmv_i32(None, None, None, None, None, None, 0, 1, 128,
-                               None) != 0
+    assert lib.ev_spmv_i32(None, None, None, None, None, None, 0, 1, 128,
+                           None) != 0
 
 
 def test_stride_rule():

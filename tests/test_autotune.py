@@ -11,10 +11,12 @@ from flybrain_amd import _lib as L
 from flybrain_amd import autotune as T
 
 
-def fake(batch, fp="c0ffee"):
+def fake(batch, fp="c0ffee", seed=0):
+    from flybrain_amd import workloads as WL
+
     net = SimpleNamespace(device=torch.device("cpu"),
                           conn=SimpleNamespace(fingerprint=lambda: fp))
-    wl = SimpleNamespace(name="w", batch=batch)
+    wl = WL.trials_b64(500, seed=seed, batch=batch, steps=100)
     return net, wl
 
 
@@ -61,3 +63,15 @@ def test_unwritable_cache_is_skipped(tmp_path, monkeypatch):
     res = T.tune(net, wl, timer=lambda *a: 5.0)
     assert res["best"] == {"road": "push", "pull_tile": None,
                            "ms_per_window": 5.0}
+
+
+def test_cache_key_covers_library_and_workload_content():
+    """A choice is not reused for another build of the library or for a
+    workload of the same name and batch with other inputs."""
+    net, wl = fake(64)
+    _, other = fake(64, seed=1)
+    assert wl.name == other.name and wl.batch == other.batch
+    assert T.cache_key(net, wl) != T.cache_key(net, other)
+    assert T.cache_key(net, wl) == T.cache_key(net, fake(64)[1])
+    assert T.library_fingerprint() in T.cache_key(net, wl)
+    assert T.library_fingerprint().startswith("abi%d-" % L.ABI_VERSION)

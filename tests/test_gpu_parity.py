@@ -296,12 +296,30 @@ def test_rates_drive_matches_table():
     torch.testing.assert_close(a.v, b.v, rtol=0, atol=0)
 
 
-def test_ev_spmv_twin_matches_dense():
-    """fis_ev_spmv_i32/i64 (the reference's ev_spmv ABI) against a dense
-    float64 product on the CPU."""
+def test_ev_spmv_binds_by_the_reference_names():
+    """libfis.so stands where the reference's ev_spmv DLL stood: it is
+    opened with a fresh ctypes.CDLL, checked with ev_spmv_abi() == 3 and its
+    ev_spmv_i32 / ev_spmv_i64 are bound by those names with the argtypes of
+    lab/flybrain/evspmv.py:886-903, then called the way Events.mm calls them
+    (evspmv.py:1115-1138: out zeroed, raw data_ptr() ints, the current
+    stream's handle).  The result equals a dense float64 product on the
+    CPU."""
     import ctypes
 
     from flybrain_amd import _lib as L
+
+    dll = ctypes.CDLL(str(L.LIB_PATH))
+    dll.ev_spmv_abi.restype = ctypes.c_int
+    dll.ev_spmv_abi.argtypes = []
+    assert int(dll.ev_spmv_abi()) == 3
+    call = {}
+    for key_t, name in (("int32", "ev_spmv_i32"), ("int64", "ev_spmv_i64")):
+        fn = getattr(dll, name)
+        fn.restype = ctypes.c_int
+        fn.argtypes = ([ctypes.c_void_p] * 6
+                       + [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                          ctypes.c_void_p])
+        call[key_t] = fn
 
     conn = C.synthetic(700, None, mean_out=50, n_groups=0, seed=5)
     n, B = conn.n, 3
@@ -310,20 +328,21 @@ def test_ev_spmv_twin_matches_dense():
     mo = (rng.random((B, n)) < 0.9).astype(np.float32)
     w = conn.dense_counts().astype(np.float64)  # [post, pre]
     want = (w @ (sp * mo).T.astype(np.float64)).T
-    lib = L.lib()
-    for dt, fn in ((torch.int32, lib.fis_ev_spmv_i32),
-                   (torch.int64, lib.fis_ev_spmv_i64)):
+    for key_t, dt in (("int32", torch.int32), ("int64", torch.int64)):
         crow = torch.from_numpy(conn.offsets).to("cuda", dt)
         col = torch.from_numpy(conn.cols.astype(np.int64)).to("cuda", dt)
         val = torch.from_numpy(conn.vals.astype(np.float32)).cuda()
         spt, mot = torch.from_numpy(sp).cuda(), torch.from_numpy(mo).cuda()
-        out = torch.zeros(B, n, device="cuda")
-        p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
-        rc = fn(p(crow), p(col), p(val), p(spt), p(mot), p(out), n, B, 128,
-                ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
-        assert rc == 0
-        np.testing.assert_array_equal(out.cpu().numpy(), want.astype(np.float32))
-    assert lib.fis_ev_spmv_abi() == 3
+        out = torch.full((B, n), 7.0, device="cuda")
+        for _ in range(2):  # the persistent out, zeroed before each launch
+            out.zero_()
+            rc = call[key_t](crow.data_ptr(), col.data_ptr(), val.data_ptr(),
+                             spt.data_ptr(), mot.data_ptr(), out.data_ptr(),
+                             n, B, 128,
+                             torch.cuda.current_stream().cuda_stream)
+            assert rc == 0
+            np.testing.assert_array_equal(out.cpu().numpy(),
+                                          want.astype(np.float32))
 
 
 @pytest.mark.parametrize("batch", [1, 3, 32])
