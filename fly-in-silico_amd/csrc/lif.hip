@@ -180,6 +180,9 @@ void k_drive_table(
   const int j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= n_entries) return;
   const fis_drive_entry e = entries[j];
+  // an entry past the state names no neuron-trial: nothing to drive (the
+  // host checks the table; this keeps a bad entry from writing anywhere)
+  if (e.idx >= (uint32_t)(s.n * s.bstride)) return;
   const int i = (int)e.idx / s.bstride;
   const int b = (int)e.idx - i * s.bstride;
   const int64_t t = now(s) - origin_of(s, b);

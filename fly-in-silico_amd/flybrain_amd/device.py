@@ -462,6 +462,81 @@ class DeviceSim:
             we = self.t_end_host.reshape(-1, 64).max(axis=1)
             self.word_end.copy_(torch.from_numpy(np.ascontiguousarray(we)))
 
+    def trial_state(self, trial):
+        """One trial's column of the state at a window boundary, in a form
+        any slot of any DeviceSim on the same connectome can load
+        (:meth:`load_trial_state`): v, g, the refractory/spike byte, the
+        flags and clamp values, the spike counts, the input in flight as
+        ``inflight[j]`` = the integer synaptic input step ``t + j`` reads
+        (j < 20: the spikes of steps t-20 .. t-1, the only ones still in
+        flight), the trial's own step (``steps_done``), the steps it still
+        counts (``steps_left``) and its Philox key.  The reference keeps
+        the same quantities per engine (STATE_TENSORS, sim.py:775-846);
+        here they are per trial, for moving a running trial between
+        slots."""
+        b = int(self._trials([trial])[0])
+        n, bs = self.n, self.bstride
+
+        def col(t):
+            return t.view(n, bs)[:, b].clone()
+
+        ring = self.ring_image().view(L.RING_LEN, n, bs)[:, :, b]
+        slots = [L.read_slot(self.t + j) for j in range(L.DELAY)]
+        return {
+            "v": col(self.v), "g": col(self.g), "st": col(self.st),
+            "counts": col(self.counts) if self.counts is not None else None,
+            "inflight": ring[slots].clone(),
+            "fl": self.fl_host[:, b].copy(),
+            "vhold": self.vhold_host[:, b].copy(),
+            "steps_done": int(self.t - self.t_origin_host[b]),
+            "steps_left": int(self.t_end_host[b] - self.t),
+            "seed": int(self.seeds_host[b]),
+            "trial_id": int(self.trial_ids_host[b]),
+            "n": n,
+        }
+
+    def load_trial_state(self, trial, state):
+        """Continue a trial saved by :meth:`trial_state` in slot ``trial``
+        of this simulation, at this simulation's clock: the slot is reset
+        (nothing of its previous occupant stays in flight), the columns are
+        written, the input in flight goes into the ring slots steps t ..
+        t+19 read (on the pull road those steps also consume the ring), and
+        the trial's own clock continues where it stopped, so its Philox
+        stream and rate schedule go on unchanged."""
+        if int(state["n"]) != self.n:
+            raise ValueError("state of a %d-neuron network, this one has %d"
+                             % (state["n"], self.n))
+        b = int(self._trials([trial])[0])
+        n, bs = self.n, self.bstride
+        tt = torch.tensor([b], dtype=torch.int32, device=self.device)
+        L.check(self.lib.fis_reset_trials(self._sp, _ptr(tt), 1,
+                                          float(self.c["v0"]), _stream()),
+                "fis_reset_trials")
+        self._slot_keep = [tt]
+        for name in ("v", "g", "st"):
+            getattr(self, name).view(n, bs)[:, b].copy_(state[name])
+        if self.counts is not None and state["counts"] is not None:
+            self.counts.view(n, bs)[:, b].copy_(state["counts"])
+        ring = self.ring.view(L.RING_LEN, n, bs)
+        for j in range(L.DELAY):
+            ring[L.read_slot(self.t + j), :, b].copy_(state["inflight"][j])
+        if self.road == "pull":
+            self.ring_until = max(self.ring_until, self.t + L.DELAY)
+        self.fl_host[:, b] = state["fl"]
+        self.vhold_host[:, b] = state["vhold"]
+        self._push_flags()
+        self.vhold.copy_(torch.from_numpy(self.vhold_host.reshape(-1)))
+        self.seeds_host[b] = np.uint64(state["seed"])
+        self.seeds.copy_(torch.from_numpy(self.seeds_host.view(np.int64)))
+        self.trial_ids_host[b] = np.uint32(state["trial_id"])
+        self.trial_ids.copy_(torch.from_numpy(
+            self.trial_ids_host.view(np.int32)))
+        self.t_origin_host[b] = self.t - int(state["steps_done"])
+        self.t_end_host[b] = self.t + int(state["steps_left"])
+        self.t_origin.copy_(torch.from_numpy(self.t_origin_host))
+        self.t_end.copy_(torch.from_numpy(self.t_end_host))
+        self._push_word_end()
+
     def trial_counts(self, trial):
         """Spike counts of one trial, int64 [n] (one column, on the host)."""
         if self.counts is None:
@@ -667,6 +742,27 @@ class DeviceSim:
         bits = np.unpackbits(words.view(np.uint8), axis=1, bitorder="little")
         bits = bits[:, :self.nb].reshape(len(rows), self.n, self.bstride)
         return bits[:, :, :self.batch].transpose(0, 2, 1).astype(bool)
+
+    def trial_raster(self, trial, t_from, t_to):
+        """Spikes of one trial at steps [t_from, t_to), bool [steps, n],
+        picked out of the raster ring on the device (only that trial's bits
+        cross to the host)."""
+        if self.raster is None:
+            raise RuntimeError("built without a raster")
+        if t_to - t_from > self.raster_len or t_from < self.t - self.raster_len:
+            raise ValueError("raster rows [%d, %d) no longer held"
+                             % (t_from, t_to))
+        b = int(self._trials([trial])[0])
+        dev = self.device
+        idx = torch.arange(self.n, device=dev, dtype=torch.int64) \
+            * self.bstride + b
+        wave = idx >> 6
+        col = (wave >> 2) * L.LINE_WORDS + (wave & 3)
+        rows = torch.tensor([t % self.raster_len for t in range(t_from, t_to)],
+                            device=dev, dtype=torch.int64)
+        words = self.raster.view(self.raster_len, -1)[rows][:, col]
+        bits = torch.bitwise_right_shift(words, (idx & 63)[None, :]) & 1
+        return bits.to(torch.bool).cpu().numpy()
 
     def rec_slot(self, u):
         """The reference's rec(u) (sum of spikes of step u-1), int [B, n]."""

@@ -22,11 +22,28 @@ of trial slots:
 
 Trials never interact, so a request's result is bit-for-bit the result of
 the same request run alone; ``tests/test_gpu_serve.py`` checks this.
+
+Around the loop, as the reference's job protocol has it (Submit / Cancel,
+lab/flybrain/ipc.py:130-152; a failing job reported without stopping the
+worker, lab/flybrain/worker.py:495-501; cancellation at a bin boundary with
+the result truncated to what ran, lab/flybrain/jobs.py:348-412):
+
+  * every request is checked when it is submitted (steps, drive and
+    silenced indices, drive size, rate); a bad one comes back as a result
+    with ``error`` set and never reaches a slot or the device;
+  * ``cancel(rid)`` takes effect at the next window boundary: a queued
+    request is dropped, a running one is read out with the counts of the
+    steps it ran (``cancelled``, ``steps_done``) and its slot freed;
+  * ``evict(rid)`` saves a running request's trial state (its column of
+    v, g, refractory counter, flags, counts and the input in flight) and
+    frees the slot; ``resume(rid)`` admits it again -- into any free slot,
+    of any shard -- and it continues bit for bit as if never interrupted.
 """
 
 from __future__ import annotations
 
 import contextlib
+import threading
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -55,17 +72,61 @@ class SimRequest:
 @dataclass
 class SimResult:
     rid: int
-    counts: np.ndarray  # int64 [n]: spikes per neuron over the request
+    counts: np.ndarray  # int64 [n]: spikes per neuron over the steps run
     steps: int
     arrival_s: float
     admitted_s: float
     done_s: float
     slot: int
+    # steps actually run: ``steps`` unless cancelled (then what ran before
+    # the cancel took effect, the reference's truncation, jobs.py:348-364)
+    steps_done: int = -1
+    cancelled: bool = False
+    error: str | None = None  # set for a rejected request (counts None)
+
+    def __post_init__(self):
+        if self.steps_done < 0:
+            self.steps_done = self.steps if self.error is None else 0
+
+    @property
+    def ok(self):
+        return self.error is None and not self.cancelled
 
     @property
     def latency_s(self):
         """Completion latency: result on the host minus arrival."""
         return self.done_s - self.arrival_s
+
+
+def check_request(req, n, max_drive):
+    """Why ``req`` cannot run on an ``n``-neuron network with ``max_drive``
+    drive slots, or None.  Checked when the request is submitted, before
+    any slot or device buffer is touched."""
+    try:
+        steps = int(req.steps)
+        if steps != req.steps or steps < 1:
+            return "steps must be a positive integer (got %r)" % (req.steps,)
+        for name in ("drive", "silenced"):
+            a = np.asarray(getattr(req, name))
+            if a.size and (a.ndim != 1 or not np.issubdtype(a.dtype,
+                                                            np.integer)):
+                return "%s must be a 1-D array of neuron indices" % name
+            if a.size and (int(a.min()) < 0 or int(a.max()) >= n):
+                return "%s index outside [0, %d)" % (name, n)
+        k = np.unique(np.asarray(req.drive, np.int64)).size
+        if k > max_drive:
+            return "drives %d neurons, more than max_drive = %d" % (k,
+                                                                    max_drive)
+        rate = float(req.rate_hz)
+        if not np.isfinite(rate):
+            return "rate_hz must be a finite number (got %r)" % (req.rate_hz,)
+        check_rates_hz(np.array([rate]), "request %r" % (req.rid,))
+        seed = int(req.seed)
+        if not 0 <= seed < (1 << 64):
+            return "seed outside [0, 2^64)"
+    except (TypeError, ValueError) as e:  # RateTooHigh is a ValueError
+        return str(e)
+    return None
 
 
 class Scheduler:
@@ -83,6 +144,8 @@ class Scheduler:
         self.free = list(range(self.slots))
         self.active = {}  # slot -> (request, t_start, admitted_s)
         self.queue = deque()
+        # evicted requests asked back: admitted before new arrivals
+        self.resumed = deque()  # (request, steps_done, admitted_s)
 
     def submit(self, req):
         """Queue a request; the queue stays in arrival order (ties in
@@ -95,14 +158,50 @@ class Scheduler:
         self.queue.insert(k, req)
 
     def admit(self, now_s, t):
+        """Resumed requests first, then arrived ones, into the lowest free
+        slots.  Returns (slot, request, steps_done) with steps_done 0 for a
+        fresh start; a resumed request keeps its own clock (t_start =
+        t - steps_done)."""
         out = []
+        while self.free and self.resumed:
+            req, done, adm = self.resumed.popleft()
+            slot = min(self.free)
+            self.free.remove(slot)
+            self.active[slot] = (req, int(t) - int(done), adm)
+            out.append((slot, req, int(done)))
         while self.free and self.queue and self.queue[0].arrival_s <= now_s:
             req = self.queue.popleft()
             slot = min(self.free)
             self.free.remove(slot)
             self.active[slot] = (req, int(t), float(now_s))
-            out.append((slot, req))
+            out.append((slot, req, 0))
         return out
+
+    def release(self, slot):
+        """Take a running request out of its slot (cancel, evict); returns
+        (request, t_start, admitted_s)."""
+        got = self.active.pop(slot)
+        self.free.append(slot)
+        return got
+
+    def drop_queued(self, rid):
+        """Remove a request that has not started; True if it was queued."""
+        for q in (self.queue,):
+            for k, req in enumerate(q):
+                if req.rid == rid:
+                    del q[k]
+                    return True
+        for k, (req, _, _) in enumerate(self.resumed):
+            if req.rid == rid:
+                del self.resumed[k]
+                return True
+        return False
+
+    def slot_of(self, rid):
+        for slot, (req, _, _) in self.active.items():
+            if req.rid == rid:
+                return slot
+        return None
 
     def finished(self, t):
         done = [s for s, (req, t0, _) in self.active.items()
@@ -120,6 +219,10 @@ class Scheduler:
     @property
     def busy(self):
         return bool(self.active)
+
+    @property
+    def pending(self):
+        return bool(self.queue or self.resumed)
 
 
 def bytes_per_trial(n, road):
@@ -194,6 +297,10 @@ class Server:
     road, one launch per window) measured best on the full connectome:
     p50 33 ms and p99 73 ms at 200 requests/s, against 60 / 155 ms with 32
     and 115 / 266 ms with 64 (pull road).
+
+    ``cancel``, ``evict`` and ``resume`` may be called from any thread (or
+    from the ``on_window`` / ``on_result`` callbacks of :meth:`serve`);
+    they take effect at the next window boundary of the serving loop.
     """
 
     def __init__(self, net: DeviceNet, slots=256, window=20, road="auto",
@@ -221,6 +328,13 @@ class Server:
         self.t = 0  # the server's clock: steps run since it started
         self._graph = None
         self.graph_windows = int(graph_windows)
+        self._commands = deque()  # (what, rid), applied between windows
+        self._lock = threading.Lock()
+        # evicted requests: rid -> (request, trial state, admitted_s)
+        self.suspended = {}
+        self._results = None
+        self._start = None
+        self._on_result = None
 
     @staticmethod
     def capacity(net, road="auto", hbm_fraction=0.5):
@@ -230,6 +344,80 @@ class Server:
 
     def _where(self, slot):
         return self.shards[slot // self.shard], slot % self.shard
+
+    # ------------------------------------------------------------ control
+    def cancel(self, rid):
+        """Stop request ``rid`` at the next window boundary: dropped if it
+        has not started, else its result holds the counts of the steps
+        that ran (``cancelled``, ``steps_done``)."""
+        with self._lock:
+            self._commands.append(("cancel", rid))
+
+    def evict(self, rid):
+        """Take running request ``rid`` out of its slot at the next window
+        boundary, keeping its trial state in ``suspended`` (no result)."""
+        with self._lock:
+            self._commands.append(("evict", rid))
+
+    def resume(self, rid):
+        """Admit suspended request ``rid`` again (before new arrivals, into
+        any free slot); it continues from where it was evicted."""
+        with self._lock:
+            self._commands.append(("resume", rid))
+
+    def _apply_commands(self, now):
+        with self._lock:
+            cmds = list(self._commands)
+            self._commands.clear()
+        for what, rid in cmds:
+            if what == "resume":
+                if rid in self.suspended:
+                    req, state, adm = self.suspended[rid]
+                    self.sched.resumed.append((req, state["steps_done"],
+                                               adm))
+                continue
+            slot = self.sched.slot_of(rid)
+            if what == "cancel":
+                if slot is not None:
+                    req, t0, adm = self.sched.release(slot)
+                    sh, local = self._where(slot)
+                    counts = sh.sim.trial_counts(local)
+                    self._emit(SimResult(
+                        req.rid, counts, req.steps, req.arrival_s, adm,
+                        time.perf_counter() - self._start, slot,
+                        steps_done=self.t - t0, cancelled=True))
+                    self._retire(slot)
+                elif rid in self.suspended:  # counts up to the eviction
+                    req, state, adm = self.suspended.pop(rid)
+                    for k, (r, _, _) in enumerate(self.sched.resumed):
+                        if r.rid == rid:
+                            del self.sched.resumed[k]
+                            break
+                    self._emit(SimResult(
+                        rid, state["counts"].to(torch.int64).cpu().numpy(),
+                        req.steps, req.arrival_s, adm,
+                        time.perf_counter() - self._start, -1,
+                        steps_done=state["steps_done"], cancelled=True))
+                else:
+                    req = next((r for r in self.sched.queue if r.rid == rid),
+                               None)
+                    if req is not None and self.sched.drop_queued(rid):
+                        self._emit(SimResult(
+                            rid, np.zeros(self.net.n, np.int64), req.steps,
+                            req.arrival_s, now,
+                            time.perf_counter() - self._start, -1,
+                            steps_done=0, cancelled=True))
+            elif what == "evict" and slot is not None:
+                req, t0, adm = self.sched.release(slot)
+                sh, local = self._where(slot)
+                state = sh.sim.trial_state(local)
+                self.suspended[req.rid] = (req, state, adm)
+                self._retire(slot)
+
+    def _emit(self, res):
+        self._results.append(res)
+        if self._on_result is not None:
+            self._on_result(res)
 
     def _capture(self):
         """One HIP graph of ``graph_windows`` windows of every shard.  The
@@ -253,10 +441,13 @@ class Server:
 
     def _advance(self):
         """Run one window of every shard holding a live request -- or, when
-        no slot is free (nothing could be admitted meanwhile) and no request
-        ends inside it, one whole captured graph of windows."""
+        no slot is free (nothing could be admitted meanwhile), no request
+        ends inside it, no control command is waiting and no shard still
+        consumes a resumed trial's input from its ring, one whole captured
+        graph of windows."""
         per = self.graph_windows * self.window
-        if per and not self.sched.free:
+        if (per and not self.sched.free and not self._commands
+                and all(sh.sim.t >= sh.sim.ring_until for sh in self.shards)):
             left = min(t0 + req.steps for req, t0, _ in
                        self.sched.active.values()) - self.t
             if left >= per:
@@ -275,32 +466,38 @@ class Server:
 
     # ------------------------------------------------------------ slots
     def _admit(self, admits):
+        """Start (or resume) admitted requests in their slots.  Requests
+        were checked at submission, so nothing here can fail on a bad
+        request."""
         by_shard = {}
-        for slot, req in admits:
+        for slot, req, done in admits:
             drv = np.unique(np.asarray(req.drive, np.int64))
-            check_rates_hz(np.array([req.rate_hz]), "request %r" % (req.rid,))
-            if drv.size > self.max_drive:
-                raise ValueError("request %r drives %d neurons > max_drive"
-                                 % (req.rid, drv.size))
             sh, local = self._where(slot)
-            by_shard.setdefault(id(sh), (sh, []))[1].append((local, req, drv))
+            by_shard.setdefault(id(sh), (sh, []))[1].append(
+                (local, req, drv, done))
         for sh, items in by_shard.values():
             s = sh.sim
-            out_ne, out_tr, free_ne, free_tr = [], [], [], []
-            for local, req, drv in items:
-                sil = np.unique(np.asarray(req.silenced, np.int64))
-                out_ne.append(sil)
-                out_tr.append(np.full(sil.size, local))
-                free_ne.append(drv)
-                free_tr.append(np.full(drv.size, local))
+            fresh = [(lo, r, d) for lo, r, d, done in items if done == 0]
+            for local, req, drv, done in items:
                 sh.set_slot_drive(local, drv, req.rate_hz)
-            s.restart_trials([lo for lo, _, _ in items],
-                             [int(r.seed) for _, r, _ in items],
-                             [s.t + int(r.steps) for _, r, _ in items],
-                             out_neurons=np.concatenate(out_ne),
-                             out_trials=np.concatenate(out_tr),
-                             free_neurons=np.concatenate(free_ne),
-                             free_trials=np.concatenate(free_tr))
+                if done:  # a resumed request: its saved column goes back
+                    _, state, _ = self.suspended.pop(req.rid)
+                    s.load_trial_state(local, state)
+            if fresh:
+                out_ne, out_tr, free_ne, free_tr = [], [], [], []
+                for local, req, drv in fresh:
+                    sil = np.unique(np.asarray(req.silenced, np.int64))
+                    out_ne.append(sil)
+                    out_tr.append(np.full(sil.size, local))
+                    free_ne.append(drv)
+                    free_tr.append(np.full(drv.size, local))
+                s.restart_trials([lo for lo, _, _ in fresh],
+                                 [int(r.seed) for _, r, _ in fresh],
+                                 [s.t + int(r.steps) for _, r, _ in fresh],
+                                 out_neurons=np.concatenate(out_ne),
+                                 out_trials=np.concatenate(out_tr),
+                                 free_neurons=np.concatenate(free_ne),
+                                 free_trials=np.concatenate(free_tr))
             sh.push_rates()
             sh.live += len(items)
 
@@ -312,21 +509,36 @@ class Server:
         sh.push_rates()
 
     # ------------------------------------------------------------ loop
-    def serve(self, requests, realtime=True, on_result=None):
+    def serve(self, requests, realtime=True, on_result=None, on_window=None):
         """Serve ``requests`` (arrival_s relative to the call) and return
         (results in completion order, stats).  With ``realtime`` the loop
         admits a request only once its arrival time has passed on the wall
-        clock; otherwise arrival times are ignored (all queued at 0)."""
+        clock; otherwise arrival times are ignored (all queued at 0).
+
+        A request that fails :func:`check_request` is answered at once
+        with an ``error`` result.  ``on_window(server)`` is called at every
+        window boundary, before commands are applied and requests
+        admitted.  Requests evicted and not resumed stay in
+        ``suspended`` when the call returns."""
+        self._results = results = []
+        self._on_result = on_result
+        self._start = start = time.perf_counter()
         reqs = sorted(requests, key=lambda r: r.arrival_s)
         for r in reqs:
+            why = check_request(r, self.net.n, self.max_drive)
+            if why is not None:
+                self._emit(SimResult(r.rid, None, r.steps, r.arrival_s,
+                                     r.arrival_s, 0.0, -1, error=why))
+                continue
             if not realtime:
                 r.arrival_s = 0.0
             self.sched.submit(r)
-        results = []
-        start = time.perf_counter()
         busy_windows = 0
-        while self.sched.queue or self.sched.busy:
+        while (self.sched.pending or self.sched.busy or self._commands):
             now = time.perf_counter() - start
+            if on_window is not None:
+                on_window(self)
+            self._apply_commands(now)
             admits = self.sched.admit(now, self.t)
             if admits:
                 self._admit(admits)
@@ -334,27 +546,37 @@ class Server:
                 nxt = self.sched.next_arrival()
                 if nxt is not None:
                     time.sleep(max(0.0, min(nxt - now, 0.001)))
+                elif not self._commands:
+                    break  # only suspended requests are left
                 continue
             busy_windows += self._advance()
             for slot, req, t0, adm in self.sched.finished(self.t):
                 sh, local = self._where(slot)
                 counts = sh.sim.trial_counts(local)  # synchronises
-                res = SimResult(req.rid, counts, req.steps, req.arrival_s,
-                                adm, time.perf_counter() - start, slot)
-                results.append(res)
+                self._emit(SimResult(req.rid, counts, req.steps,
+                                     req.arrival_s, adm,
+                                     time.perf_counter() - start, slot))
                 self._retire(slot)
-                if on_result is not None:
-                    on_result(res)
         torch.cuda.synchronize()
         wall = time.perf_counter() - start
+        self._on_result = None
         return results, summarize(results, wall, self.slots, busy_windows,
                                   self.window)
 
 
 def summarize(results, wall, slots, windows, window):
-    lat = np.array([r.latency_s for r in results]) if results else np.zeros(1)
-    steps = sum(r.steps for r in results)
-    return {"requests": len(results), "slots": int(slots),
+    """Throughput and completion latency of the requests that ran to the
+    end; rejected and cancelled ones are counted apart (cancelled steps
+    still count as simulated work)."""
+    done = [r for r in results if getattr(r, "ok", True)]
+    lat = np.array([r.latency_s for r in done]) if done else np.zeros(1)
+    steps = sum(r.steps_done if hasattr(r, "steps_done") else r.steps
+                for r in results if getattr(r, "error", None) is None)
+    return {"requests": len(done), "slots": int(slots),
+            "rejected": sum(1 for r in results
+                            if getattr(r, "error", None) is not None),
+            "cancelled": sum(1 for r in results
+                             if getattr(r, "cancelled", False)),
             "wall_s": round(wall, 4),
             "trial_s_per_s": round(steps * 1e-4 / wall, 3) if wall else 0.0,
             "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
@@ -395,4 +617,5 @@ def ctypes_stream():
 
 
 __all__ = ["SimRequest", "SimResult", "Scheduler", "Server", "NO_END",
-           "bytes_per_trial", "poisson_requests", "summarize"]
+           "bytes_per_trial", "check_request", "poisson_requests",
+           "summarize"]

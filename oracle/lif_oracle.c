@@ -20,11 +20,15 @@
  *                       generator
  *   oracle_philox    -- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11,
  *                       "Parallel random numbers: as easy as 1, 2, 3")
+ *   oracle_run_counts -- oracle_step looped over a whole run of one trial
+ *                       fed a sparse tape (spike counts only; for long
+ *                       statistical comparisons)
  *
  * Arrays are trial-major [B][N] like the reference's tensors; the delay
  * ring is [B][L][N] like Engine.delay_buffer (sim.py:385-386).
  */
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 
 /* ------------------------------------------------------------ Philox */
@@ -197,4 +201,77 @@ int oracle_step(int n, int batch, const int64_t* offsets, const int32_t* cols,
   return rc;
 }
 
-int oracle_abi(void) { return 1; }
+/* ------------------------------------------------------------ helpers */
+
+/* The 24-bit uniforms (x >> 8) the drive compares with p, for neurons
+ * n0 .. n0+n-1 and steps t0 .. t0+steps-1 of one trial:
+ * out[(t - t0) * n + (i - n0)].  For statistical checks of the stream. */
+void oracle_uniform_grid(uint64_t seed, uint32_t trial_id, uint32_t n0,
+                         int n, uint32_t t0, int steps, uint32_t* out) {
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int k = 0; k < steps; ++k) {
+    uint32_t t = t0 + (uint32_t)k;
+    for (int i = 0; i < n; ++i) {
+      uint32_t ctr[4] = {t >> 2, n0 + (uint32_t)i, trial_id, 0u};
+      uint32_t x[4];
+      oracle_philox(ctr, key, x);
+      out[(size_t)k * n + i] = x[t & 3u] >> 8;
+    }
+  }
+}
+
+/* One trial (B = 1) from rest (v = v0, refrac = refrac_steps, nothing in
+ * flight) for `steps` steps, kicked by a sparse tape: at step t the
+ * neurons tape_idx[tape_ptr[t] .. tape_ptr[t+1]) get the Poisson stim
+ * w_scale * (1 * gain) (sim.py:859-861), the others w_scale * (0 * gain).
+ * counts[i] (int64) receives the spikes of neuron i.  Returns 0, -1 if an
+ * input sum reached 2^24, -2 if out of memory. */
+int oracle_run_counts(int n, int steps, const int64_t* offsets,
+                      const int32_t* cols, const int16_t* vals,
+                      const oracle_params* prm, const float* refrac_steps,
+                      float v0, const int64_t* tape_ptr,
+                      const int32_t* tape_idx, float gain, int L,
+                      int64_t* counts) {
+  size_t nn = (size_t)n;
+  float* v = malloc(sizeof(float) * nn);
+  float* g = calloc(nn, sizeof(float));
+  float* sp = calloc(nn, sizeof(float));
+  float* rf = malloc(sizeof(float) * nn);
+  float* ring = calloc(nn * (size_t)L, sizeof(float));
+  float* stim = malloc(sizeof(float) * nn);
+  int64_t* acc = malloc(sizeof(int64_t) * nn);
+  int rc = 0;
+  if (!v || !g || !sp || !rf || !ring || !stim || !acc) {
+    rc = -2;
+    goto done;
+  }
+  const float off = prm->w_scale * (0.0f * gain);
+  const float on = prm->w_scale * (1.0f * gain);
+  for (size_t i = 0; i < nn; ++i) {
+    v[i] = v0;
+    rf[i] = refrac_steps[i];
+    stim[i] = off;
+    counts[i] = 0;
+  }
+  for (int t = 0; t < steps; ++t) {
+    for (int64_t e = tape_ptr[t]; e < tape_ptr[t + 1]; ++e)
+      stim[tape_idx[e]] = on;
+    if (oracle_step(n, 1, offsets, cols, vals, prm, v, g, sp, rf, ring, L,
+                    t % L, refrac_steps, 0, 0, 0, 0, 0, 0, stim, 0, 0, acc))
+      rc = -1;
+    for (int64_t e = tape_ptr[t]; e < tape_ptr[t + 1]; ++e)
+      stim[tape_idx[e]] = off;
+    for (size_t i = 0; i < nn; ++i) counts[i] += sp[i] != 0.0f;
+  }
+done:
+  free(v);
+  free(g);
+  free(sp);
+  free(rf);
+  free(ring);
+  free(stim);
+  free(acc);
+  return rc;
+}
+
+int oracle_abi(void) { return 2; }

@@ -68,7 +68,17 @@ def lib():
                                   + [P] * 5 + [ctypes.c_int, ctypes.c_int, P,
                                                ctypes.c_int] + [P] * 9)
         L.oracle_step.restype = ctypes.c_int
+        L.oracle_uniform_grid.argtypes = [ctypes.c_uint64, ctypes.c_uint32,
+                                          ctypes.c_uint32, ctypes.c_int,
+                                          ctypes.c_uint32, ctypes.c_int, P]
+        L.oracle_uniform_grid.restype = None
+        L.oracle_run_counts.argtypes = ([ctypes.c_int, ctypes.c_int, P, P, P,
+                                         P, P, ctypes.c_float, P, P,
+                                         ctypes.c_float, ctypes.c_int, P])
+        L.oracle_run_counts.restype = ctypes.c_int
         L.oracle_abi.restype = ctypes.c_int
+        if L.oracle_abi() != 2:
+            raise RuntimeError("stale %s: rebuild the oracle" % LIB_PATH)
         _lib = L
     return _lib
 
@@ -89,6 +99,42 @@ def philox(ctr, key):
 def draw(seed, trial_id, neuron, t, p):
     return int(lib().oracle_draw(int(seed), int(trial_id), int(neuron),
                                  int(t), float(np.float32(p))))
+
+
+def uniform_grid(seed, trial_id, n0, n, t0, steps):
+    """The drive stream's 24-bit uniforms, uint32 [steps, n] (neurons n0..,
+    steps t0..): the values oracle_draw compares with p * 2^24."""
+    out = np.zeros((steps, n), np.uint32)
+    lib().oracle_uniform_grid(int(seed), int(trial_id), int(n0), int(n),
+                              int(t0), int(steps), _p(out))
+    return out
+
+
+def run_counts(conn, steps, tape_ptr, tape_idx, no_refractory=None,
+               params=None):
+    """Spike counts int64 [n] of one trial run from rest for ``steps``
+    steps with the kicks of a sparse tape (neurons tape_idx[tape_ptr[t]:
+    tape_ptr[t+1]] at step t): OracleEngine's step, looped in C.  The call
+    releases the GIL, so trials run in parallel threads."""
+    c = derived(params)
+    rs = np.full(conn.n, c["refrac_steps"], np.float32)
+    if no_refractory is not None and len(no_refractory):
+        rs[np.asarray(no_refractory, dtype=np.int64)] = 0.0
+    prm = _Params(*(float(c[k]) for k in ("w_scale", "a_syn", "a_mem",
+                                          "v_rest", "v_reset", "v_th")))
+    ptr = np.ascontiguousarray(tape_ptr, dtype=np.int64)
+    idx = np.ascontiguousarray(tape_idx, dtype=np.int32)
+    if ptr.size != steps + 1:
+        raise ValueError("tape_ptr needs steps + 1 entries")
+    counts = np.zeros(conn.n, np.int64)
+    rc = lib().oracle_run_counts(
+        conn.n, int(steps), _p(conn.offsets), _p(conn.cols), _p(conn.vals),
+        ctypes.byref(prm), _p(rs), float(c["v0"]), _p(ptr),
+        _p(idx if idx.size else np.zeros(1, np.int32)), float(c["gain"]),
+        c["buf_len"], _p(counts))
+    if rc != 0:
+        raise ArithmeticError("oracle_run_counts failed (%d)" % rc)
+    return counts
 
 
 def derived(params=None):

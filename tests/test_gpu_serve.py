@@ -98,3 +98,77 @@ def test_graph_replay_serves_the_same(net, road, slots):
     for r in graph:
         np.testing.assert_array_equal(r.counts, a[r.rid],
                                       err_msg="request %d" % r.rid)
+
+
+def _bad_requests(n):
+    base = dict(steps=80, drive=np.array([1, 2, 3]), rate_hz=300.0,
+                silenced=np.zeros(0, np.int64), seed=5)
+    return [SimRequest(rid=1000, **{**base, "drive": np.array([1, n])}),
+            SimRequest(rid=1001, **{**base, "rate_hz": 20_000.0}),
+            SimRequest(rid=1002, **{**base, "steps": 0}),
+            SimRequest(rid=1003, **{**base, "drive": np.arange(300)}),
+            SimRequest(rid=1004, **{**base, "silenced": np.array([-3])})]
+
+
+@pytest.mark.parametrize("road,slots,graph_windows", [("push", 6, 0),
+                                                      ("pull", 6, 0),
+                                                      ("push", 5, 2)])
+def test_bad_requests_cancel_and_eviction_leave_the_rest_unchanged(
+        net, road, slots, graph_windows):
+    """Rejected requests come back as errors without reaching a slot; a
+    queued request is cancelled (nothing ran), a running one is cancelled
+    with the counts of the steps it ran, another is evicted and later
+    resumed into whatever slot is free.  Every other request's counts are
+    bit-identical to a clean run of the same requests; the cancelled one
+    equals the request run alone for the steps it ran, and the evicted one
+    equals its uninterrupted result."""
+    reqs = requests(net.n, 16, seed=77)
+    for r in reqs:
+        r.steps = max(r.steps, 120)
+    clean, _ = Server(net, slots=slots, road=road,
+                      graph_windows=graph_windows).serve(
+        [SimRequest(**vars(r)) for r in reqs], realtime=False)
+    want = {r.rid: r.counts for r in clean}
+
+    srv = Server(net, slots=slots, road=road, graph_windows=graph_windows)
+    seen = {"w": 0}
+    plan = {}
+
+    def on_window(s):
+        w = seen["w"]
+        seen["w"] += 1
+        live = sorted((req.rid for req, _, _ in s.sched.active.values()))
+        if w == 1:
+            plan["queued"] = s.sched.queue[-1].rid
+            s.cancel(plan["queued"])
+        elif w == 3:
+            plan["cancel"] = live[0]
+            s.cancel(live[0])
+        elif w == 4:
+            plan["evict"] = live[-1]
+            s.evict(live[-1])
+        elif w == 9:
+            assert plan["evict"] in s.suspended
+            s.resume(plan["evict"])
+
+    mix = [SimRequest(**vars(r)) for r in reqs] + _bad_requests(net.n)
+    got, stats = srv.serve(mix, realtime=False, on_window=on_window)
+    res = {r.rid: r for r in got}
+    assert not srv.suspended
+    for k in range(1000, 1005):
+        assert res[k].error and res[k].counts is None, k
+    assert stats["rejected"] == 5 and stats["cancelled"] == 2
+    q = res[plan["queued"]]
+    assert q.cancelled and q.steps_done == 0 and q.counts.sum() == 0
+    c = res[plan["cancel"]]
+    assert c.cancelled and 0 < c.steps_done < c.steps
+    one = next(r for r in reqs if r.rid == plan["cancel"])
+    alone, _ = Server(net, slots=1, road="push").serve(
+        [SimRequest(**{**vars(one), "steps": c.steps_done})], realtime=False)
+    np.testing.assert_array_equal(c.counts, alone[0].counts)
+    for rid, counts in want.items():
+        if rid in (plan["queued"], plan["cancel"]):
+            continue
+        assert res[rid].ok
+        np.testing.assert_array_equal(res[rid].counts, counts,
+                                      err_msg="request %d" % rid)

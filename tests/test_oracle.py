@@ -106,6 +106,97 @@ def test_bernoulli_rate():
     assert abs(k - mu) < 5 * np.sqrt(mu)
 
 
+def test_uniform_grid_is_the_draw():
+    """The grid helper returns exactly the 24-bit uniforms oracle_draw
+    compares: draw(p) is u < p * 2^24, so p = u * 2^-24 gives no kick and
+    p = (u + 1) * 2^-24 a kick (pins the comparison and its scale)."""
+    u = O.uniform_grid(99, 3, 10, 6, 37, 9)
+    for k in range(9):
+        for i in range(6):
+            x = int(u[k, i])
+            assert O.draw(99, 3, 10 + i, 37 + k, np.float32(x / 2**24)) == 0
+            assert O.draw(99, 3, 10 + i, 37 + k,
+                          np.float32((x + 1) / 2**24)) == 1
+
+
+def _corr(a, b):
+    a = a.astype(np.float64).ravel() - a.mean()
+    b = b.astype(np.float64).ravel() - b.mean()
+    return float((a * b).sum() / np.sqrt((a * a).sum() * (b * b).sum()))
+
+
+def stream_checks(u, p=0.015):
+    """Uniformity and independence of a [steps, neurons] grid of 24-bit
+    uniforms: the mean, lag-1 correlations across neighbouring steps
+    (within and across the 4-word Philox blocks) and neighbouring neurons,
+    and the kick rate and the rate of kicks at two consecutive steps of a
+    Bernoulli(p) drive, each within 5 standard errors."""
+    x = u.astype(np.float64) / 2**24
+    N = x.size
+    assert abs(x.mean() - 0.5) < 5 * np.sqrt(1 / 12 / N)
+    lim = 5 / np.sqrt(N)
+    for lag_t in (1, 2, 3, 4):
+        assert abs(_corr(x[lag_t:], x[:-lag_t])) < lim, "step lag %d" % lag_t
+    assert abs(_corr(x[:, 1:], x[:, :-1])) < lim, "neuron lag 1"
+    kick = u < np.uint32(p * 2**24)
+    mu = p * N
+    assert abs(kick.sum() - mu) < 5 * np.sqrt(mu * (1 - p))
+    both = (kick[1:] & kick[:-1]).sum()
+    m2 = p * p * kick[1:].size
+    assert abs(both - m2) < 5 * np.sqrt(m2)
+
+
+def test_drive_stream_is_uniform_and_independent():
+    """The Philox drive stream over 1,500 neurons x 2,000 steps of one
+    trial, and across trials and seeds (a trial id or seed that did not
+    enter the counter or key would correlate the grids)."""
+    a = O.uniform_grid(7, 0, 0, 1500, 0, 2000)
+    stream_checks(a)
+    b = O.uniform_grid(7, 1, 0, 1500, 0, 2000)      # another trial
+    c = O.uniform_grid(8, 0, 0, 1500, 0, 2000)      # another seed
+    lim = 5 / np.sqrt(a.size)
+    assert abs(_corr(a, b)) < lim and abs(_corr(a, c)) < lim
+    assert abs(_corr(b, c)) < lim
+
+
+def test_stream_checks_catch_a_reused_word():
+    """The checks above have teeth: a stream that reused one Philox word
+    for all four steps of a block (a dropped ``t & 3``), or one that
+    ignored the neuron, fails them."""
+    a = O.uniform_grid(7, 0, 0, 1500, 0, 2000)
+    reused = a[(np.arange(a.shape[0]) // 4) * 4]
+    with pytest.raises(AssertionError):
+        stream_checks(reused)
+    same_neuron = np.repeat(a[:, :1], a.shape[1], axis=1)
+    with pytest.raises(AssertionError):
+        stream_checks(same_neuron)
+
+
+def test_run_counts_equals_the_stepped_oracle():
+    """oracle_run_counts (the C loop the long statistical comparisons use)
+    gives the spike counts of OracleEngine stepped one step at a time on
+    the same tape."""
+    conn = C.synthetic(900, None, mean_out=50, count_exp=1.3, n_groups=3,
+                       seed=2)
+    rng = np.random.default_rng(3)
+    drv = rng.choice(conn.n, 25, replace=False)
+    steps = 400
+    kicks = rng.random((steps, drv.size)) < 0.08
+    t_, j = np.nonzero(kicks)
+    ptr = np.zeros(steps + 1, np.int64)
+    np.cumsum(np.bincount(t_, minlength=steps), out=ptr[1:])
+    got = O.run_counts(conn, steps, ptr, drv[j], no_refractory=drv)
+    ora = O.OracleEngine(conn, 1, no_refractory=drv)
+    bits = np.zeros((1, conn.n), np.uint8)
+    want = np.zeros(conn.n, np.int64)
+    for t in range(steps):
+        bits[:] = 0
+        bits[0, drv[kicks[t]]] = 1
+        want += ora.step_bits(bits)[0].astype(np.int64)
+    assert want.sum() > kicks.sum()  # recurrent spikes beyond the kicks
+    np.testing.assert_array_equal(got, want)
+
+
 # --------------------------------------------------------------- closed forms
 
 
