@@ -24,8 +24,16 @@
 
 #include <climits>
 #include <cstddef>
+#include <mutex>
 
 #include "../../include/fis.h"
+
+// Event staging of fis_window_noalloc, for callers that cannot allocate:
+// one array per device, held until the process ends, shared by every call
+// on that device (calls that overlap on one GPU race).  Every word a call
+// reads from it is written earlier in the same call, so nothing in it has
+// to start at zero.
+static __device__ __align__(256) unsigned char scratch[FIS_SCRATCH_BYTES];
 
 namespace {
 
@@ -1446,18 +1454,22 @@ bool valid_stride(int32_t bs) {
   return bs % kWave == 0;
 }
 
-int check_sim(const fis_sim* s) {
+// events of a full window plus one sink slot per wave segment, rounded up
+// to whole 128-byte lines
+int min_ev_cap(int32_t bs) {
+  const int need = FIS_MAX_WINDOW * (bs >= kWave ? 1 : kWave / bs) + 1;
+  return (need + FIS_EVENT_ALIGN - 1) / FIS_EVENT_ALIGN * FIS_EVENT_ALIGN;
+}
+
+// Everything check_sim checks except the event staging (events, ev_count,
+// ev_cap), which fis_window_noalloc brings itself.
+int check_sim_state(const fis_sim* s) {
   if (!s || s->n <= 0 || s->batch <= 0 || !valid_stride(s->bstride) ||
       s->batch > s->bstride || s->bstride > 65535 + 1)
     return (int)hipErrorInvalidValue;
-  // events of a full window plus one sink slot per wave segment
-  const int need =
-      FIS_MAX_WINDOW * (s->bstride >= kWave ? 1 : kWave / s->bstride) + 1;
-  if (s->ev_cap < need || s->ev_cap % FIS_EVENT_ALIGN != 0)
-    return (int)hipErrorInvalidValue;
   if (!s->offsets || !s->cols || !s->vals || !s->v || !s->g || !s->st ||
-      !s->fl || !s->ring || !s->sbits || !s->events || !s->ev_count ||
-      !s->clock || !s->seeds || !s->trial_ids)
+      !s->fl || !s->ring || !s->sbits || !s->clock || !s->seeds ||
+      !s->trial_ids)
     return (int)hipErrorInvalidValue;
   if (s->raster && s->raster_len < FIS_MAX_WINDOW)
     return (int)hipErrorInvalidValue;
@@ -1470,6 +1482,14 @@ int check_sim(const fis_sim* s) {
   if (s->drive_group && s->drive_dense) return (int)hipErrorInvalidValue;
   if ((int64_t)s->n * s->bstride >= ((int64_t)1 << 31) - kBlock)
     return (int)hipErrorInvalidValue;
+  return 0;
+}
+
+int check_sim(const fis_sim* s) {
+  if (int rc = check_sim_state(s)) return rc;
+  if (s->ev_cap < min_ev_cap(s->bstride) || s->ev_cap % FIS_EVENT_ALIGN != 0)
+    return (int)hipErrorInvalidValue;
+  if (!s->events || !s->ev_count) return (int)hipErrorInvalidValue;
   return 0;
 }
 
@@ -1494,6 +1514,60 @@ int check_pull(const fis_sim* s) {
 
 unsigned grid_for(int64_t items) {
   return (unsigned)((items + kBlock - 1) / kBlock);
+}
+
+// W update steps of the push road, spikes into the event segments of
+// s.events / s.ev_count (checked by the caller)
+void launch_update(const fis_sim& s, int W, hipStream_t stream) {
+  const int64_t nb = (int64_t)s.n * s.bstride;
+  if (W == 1)
+    hipLaunchKernelGGL((k_update<1, false>), dim3(grid_for(nb)), dim3(kBlock),
+                       0, stream, s, W);
+  else
+    hipLaunchKernelGGL((k_update<FIS_MAX_WINDOW, false>), dim3(grid_for(nb)),
+                       dim3(kBlock), 0, stream, s, W);
+}
+
+// the window's events into the int32 ring, then clock += W (unless
+// defer_tick)
+void launch_scatter(const fis_sim& s, int W, hipStream_t stream) {
+  const int64_t nw = ((int64_t)s.n * s.bstride + kWave - 1) / kWave;
+  int64_t blocks = (nw + (kBlock / kWave) - 1) / (kBlock / kWave);
+  if (s.bstride >= kWave) {
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_scatter_steps, dim3((unsigned)blocks, (unsigned)W),
+                       dim3(kBlock), 0, stream, s, W);
+  } else {
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_scatter, dim3((unsigned)blocks), dim3(kBlock), 0,
+                       stream, s, W);
+  }
+}
+
+// ---- the static scratch of fis_window_noalloc
+//
+// Device address of `scratch` on each device, looked up once per device
+// with that device current (every GPU holds its own copy of the array at
+// its own address; the host-side name of the array is not a device
+// address and is never passed to a kernel).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kMaxDevices];
+
+int scratch_address(unsigned char** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  if (dev < 0 || dev >= kMaxDevices) return (int)hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    g_scratch[dev] = (unsigned char*)p;
+  }
+  *out = g_scratch[dev];
+  return 0;
 }
 
 }  // namespace
@@ -1594,13 +1668,7 @@ int fis_drive_tape(const fis_sim* s, const int64_t* tape_ptr,
 int fis_update(const fis_sim* s, int W, void* stream) {
   if (int rc = check_sim(s)) return rc;
   if (W < 1 || W > FIS_MAX_WINDOW) return (int)hipErrorInvalidValue;
-  const int64_t nb = (int64_t)s->n * s->bstride;
-  if (W == 1)
-    hipLaunchKernelGGL((k_update<1, false>), dim3(grid_for(nb)), dim3(kBlock),
-                       0, (hipStream_t)stream, *s, W);
-  else
-    hipLaunchKernelGGL((k_update<FIS_MAX_WINDOW, false>), dim3(grid_for(nb)),
-                       dim3(kBlock), 0, (hipStream_t)stream, *s, W);
+  launch_update(*s, W, (hipStream_t)stream);
   return (int)hipGetLastError();
 }
 int fis_tick(const fis_sim* s, int steps, void* stream) {
@@ -1630,17 +1698,30 @@ int fis_update_scatter(const fis_sim* s, int W, void* stream) {
 int fis_scatter(const fis_sim* s, int W, void* stream) {
   if (int rc = check_sim(s)) return rc;
   if (W < 1 || W > FIS_MAX_WINDOW) return (int)hipErrorInvalidValue;
-  const int64_t nw = fis_n_waves(s->n, s->bstride);
-  int64_t blocks = (nw + (kBlock / kWave) - 1) / (kBlock / kWave);
-  if (s->bstride >= kWave) {
-    if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(k_scatter_steps, dim3((unsigned)blocks, (unsigned)W),
-                       dim3(kBlock), 0, (hipStream_t)stream, *s, W);
-  } else {
-    if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(k_scatter, dim3((unsigned)blocks), dim3(kBlock), 0,
-                       (hipStream_t)stream, *s, W);
-  }
+  launch_scatter(*s, W, (hipStream_t)stream);
+  return (int)hipGetLastError();
+}
+
+int fis_window_noalloc(const fis_sim* s, int W, void* stream) {
+  if (int rc = check_sim_state(s)) return rc;
+  if (W < 1 || W > FIS_MAX_WINDOW) return (int)hipErrorInvalidValue;
+  // the staging this window needs: the update's event segments, then one
+  // 128-byte line of event counts per update workgroup
+  fis_sim t = *s;
+  t.ev_cap = min_ev_cap(s->bstride);
+  const int64_t ev_bytes =
+      (fis_n_waves(s->n, s->bstride) * t.ev_cap * (int64_t)sizeof(fis_event) +
+       255) / 256 * 256;
+  const int64_t cnt_bytes =
+      fis_n_blocks(s->n, s->bstride) * FIS_LINE_INTS * (int64_t)sizeof(int32_t);
+  if (ev_bytes + cnt_bytes > (int64_t)FIS_SCRATCH_BYTES)
+    return (int)hipErrorInvalidValue;  // larger than the static scratch
+  unsigned char* base = nullptr;
+  if (int rc = scratch_address(&base)) return rc;
+  t.events = (fis_event*)base;
+  t.ev_count = (int32_t*)(base + ev_bytes);
+  launch_update(t, W, (hipStream_t)stream);
+  launch_scatter(t, W, (hipStream_t)stream);
   return (int)hipGetLastError();
 }
 

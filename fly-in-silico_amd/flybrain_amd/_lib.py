@@ -20,9 +20,10 @@ CSRC = PKG.parent / "csrc"
 SOURCES = [CSRC / "lif.hip"]
 HEADER = ROOT / "include" / "fis.h"
 LIB_PATH = PKG / "libfis.so"
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 MAX_WINDOW = 20
+SCRATCH_BYTES = 64 << 20  # FIS_SCRATCH_BYTES: fis_window_noalloc's staging
 RING_LEN = 40  # int32 ring slots (two delays)
 DELAY = 20     # steps from a spike to the input it makes
 
@@ -146,6 +147,7 @@ def lib():
         "fis_drive_tape": ([sims, P, P, I64, I, P], I),
         "fis_update": ([sims, I, P], I),
         "fis_scatter": ([sims, I, P], I),
+        "fis_window_noalloc": ([sims, I, P], I),
         "fis_update_scatter": ([sims, I, P], I),
         "fis_tick": ([sims, I, P], I),
         "fis_seed_ring": ([sims, I, P], I),

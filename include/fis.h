@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define FIS_ABI_VERSION 10
+#define FIS_ABI_VERSION 11
 #define FIS_MAX_WINDOW 20   /* steps per window: delay horizon 19 + 1     */
 #define FIS_RING_LEN 40     /* int32 ring slots, two delays: the spikes of
                                step u are summed into slot (u + 21) mod 40,
@@ -259,6 +259,26 @@ int fis_scatter(const fis_sim* s, int W, void* stream);
  * step of the window reads (Engine._core + the SpMV, sim.py:906-953); a
  * one-thread launch then advances the clock by W (unless defer_tick). */
 int fis_update_scatter(const fis_sim* s, int W, void* stream);
+
+/* Static scratch of fis_window_noalloc: 64 MiB per device. */
+#define FIS_SCRATCH_BYTES (64u << 20)
+
+/* fis_update + fis_scatter as one call whose event staging (the per-wave
+ * event segments and counts that fis_update writes and fis_scatter reads)
+ * lives in a static device array of FIS_SCRATCH_BYTES that the library
+ * holds on each GPU it runs on, instead of s->events / s->ev_count (which
+ * may be NULL here; s->ev_cap is ignored).  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); it calls no allocator.  Push road, any trial stride.  The
+ * staging needs n_waves * ev_cap * 16 + n_blocks * 128 bytes (ev_cap =
+ * 20 * max(1, 64 / bstride) + 1 rounded up to 8): at most about 10 M
+ * neuron-trials from 64 trials up, 2.9 M at 16, 0.2 M at 1 (the
+ * 138,639-neuron network fits at B = 1, 16 and 64); a larger simulation
+ * returns hipErrorInvalidValue without launching.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams,
+ * other threads) race.  Same results as fis_update then fis_scatter. */
+int fis_window_noalloc(const fis_sim* s, int W, void* stream);
 
 /* *clock += steps (one thread): closes a run of windows launched with
  * defer_tick set.  No counterpart in the reference, whose step counter is
