@@ -266,6 +266,41 @@ def to_flywire_tables(conn: Connectome, conn_parquet, completeness_csv=None):
                      ).to_csv(completeness_csv)
 
 
+def default_paths(root=None):
+    """Where the reference keeps its connectome (lab/flybrain/data.py:
+    31-48): ``cache/weights_coo.pt`` and ``upstream/data/2025_*_783``
+    under its project root -- ``root``, else ``$FLYBRAIN_ROOT``, else the
+    current directory."""
+    import os
+    from pathlib import Path
+
+    base = Path(root or os.environ.get("FLYBRAIN_ROOT") or ".")
+    return {"cache": base / "cache" / "weights_coo.pt",
+            "connectivity": base / "upstream" / "data"
+            / "2025_Connectivity_783.parquet",
+            "completeness": base / "upstream" / "data"
+            / "2025_Completeness_783.csv"}
+
+
+def load_default(root=None) -> Connectome:
+    """The reference's default connectome (data.load_weights,
+    lab/flybrain/data.py:221-259): its COO cache if present (loaded with
+    ``weights_only=True``), else the FlyWire connectivity table and the
+    completeness CSV.  Raises FileNotFoundError naming the paths when
+    neither is there."""
+    p = default_paths(root)
+    if p["cache"].exists():
+        import torch
+
+        return from_reference_coo(torch.load(p["cache"], weights_only=True))
+    if p["connectivity"].exists() and p["completeness"].exists():
+        return from_flywire_tables(p["connectivity"], p["completeness"])
+    raise FileNotFoundError(
+        "no connectome given and none at the reference's default locations "
+        "(%s, or %s with %s); pass weights, or set FLYBRAIN_ROOT"
+        % (p["cache"], p["connectivity"], p["completeness"]))
+
+
 def save(conn: Connectome, path):
     """Write the by-pre CSR as an uncompressed .npz."""
     extra = {} if conn.groups is None else {"groups": conn.groups}

@@ -691,6 +691,34 @@ class DeviceSim:
                 raise TypeError("unknown drive %r" % type(drive))
         self.step_launch(W)
 
+    def read_kicks(self):
+        """The Poisson kicks a drive launch has set for the window starting
+        now (before the update consumes them), as (steps, neurons, trials)
+        int64 arrays: the window's slice of a tape."""
+        sb = self.sbits.view(self.n, self.bstride)[:, :self.batch] \
+            .cpu().numpy().view(np.uint32)
+        ne, tr = np.nonzero(sb)
+        words = np.ascontiguousarray(sb[ne, tr])
+        bits = np.unpackbits(words.view(np.uint8).reshape(-1, 4), axis=1,
+                             bitorder="little")
+        row, k = np.nonzero(bits)
+        return ((self.t + k).astype(np.int64), ne[row].astype(np.int64),
+                tr[row].astype(np.int64))
+
+    def record_window(self, W, drive):
+        """One window driven by ``drive`` (a DriveTable, or rates [batch, n]
+        on the device), returning its kicks (:meth:`read_kicks`) -- a tape
+        of the Philox drive that replays the same window exactly."""
+        if isinstance(drive, torch.Tensor):
+            self.drive_rates(drive, W)
+        elif isinstance(drive, DriveTable):
+            self.drive_table(drive, W)
+        else:
+            raise TypeError("record a DriveTable or a rates tensor")
+        kicks = self.read_kicks()
+        self.step_launch(W)
+        return kicks
+
     def seed_ring(self, slot):
         L.check(self.lib.fis_seed_ring(self._sp, slot, _stream()),
                 "fis_seed_ring")
@@ -820,8 +848,103 @@ class DriveTable:
         self.rate_hz.copy_(torch.from_numpy(rt.reshape(-1)))
 
 
+TAPE_FORMAT = "flybrain_amd.tape/1"
+TRIAL_STATE_FORMAT = "flybrain_amd.trial_state/1"
+
+
+def save_trial_state(path, state):
+    """Write a :meth:`DeviceSim.trial_state` dict as ``.npz`` (no pickled
+    objects), so an evicted trial can be resumed by another process."""
+    arrays = {"format": np.array(TRIAL_STATE_FORMAT)}
+    for k, v in state.items():
+        if isinstance(v, torch.Tensor):
+            arrays[k] = v.cpu().numpy()
+        elif v is None:
+            continue
+        else:
+            arrays[k] = np.asarray(v)
+    with open(path, "wb") as f:
+        np.savez_compressed(f, **arrays)
+
+
+def load_trial_state(path):
+    """Read a trial state written by :func:`save_trial_state` (tensors on
+    the host; :meth:`DeviceSim.load_trial_state` copies them over)."""
+    with np.load(path, allow_pickle=False) as z:
+        if "format" not in z.files or str(z["format"]) != TRIAL_STATE_FORMAT:
+            raise ValueError("%s is not a %s file" % (path,
+                                                      TRIAL_STATE_FORMAT))
+        out = {"counts": None}
+        for k in z.files:
+            if k == "format":
+                continue
+            a = z[k]
+            if k in ("fl", "vhold"):
+                out[k] = a.copy()
+            elif a.ndim == 0:
+                out[k] = int(a)
+            else:
+                out[k] = torch.from_numpy(a.copy())
+        return out
+
+
+def save_tape(path, steps, neurons, trials, n, batch, seeds=None):
+    """Write a Poisson tape as ``.npz`` (no pickled objects): the kicked
+    (step, neuron, trial) triples, the shape [n_steps, batch, n] and,
+    when the tape was recorded from the Philox drive, the trials' seeds.
+    The reference's drive is Bernoulli(rate * DT/1000) per neuron-trial
+    and step (lab/flybrain/sim.py:848-861); a tape is its outcomes, so a
+    run fed the tape is reproducible by any engine of the same step --
+    the reference's own included."""
+    st = np.asarray(steps, np.int64).reshape(-1)
+    ne = np.asarray(neurons, np.int64).reshape(-1)
+    tr = np.asarray(trials, np.int64).reshape(-1)
+    if not (st.size == ne.size == tr.size):
+        raise ValueError("steps, neurons and trials differ in length")
+    if ne.size and (ne.min() < 0 or ne.max() >= n or tr.min() < 0
+                    or tr.max() >= batch or st.min() < 0):
+        raise IndexError("tape event outside [steps, %d trials, %d neurons]"
+                         % (batch, n))
+    n_steps = int(st.max()) + 1 if st.size else 0
+    order = np.lexsort((ne, tr, st))
+    arrays = {"format": np.array(TAPE_FORMAT),
+              "step": st[order], "neuron": ne[order].astype(np.int32),
+              "trial": tr[order].astype(np.int32),
+              "shape": np.array([n_steps, batch, n], np.int64)}
+    if seeds is not None:
+        arrays["seed"] = np.asarray(seeds, np.uint64).reshape(-1)
+    with open(path, "wb") as f:
+        np.savez_compressed(f, **arrays)
+
+
+def load_tape(path):
+    """Read a tape written by :func:`save_tape`: a dict with ``step``,
+    ``neuron``, ``trial`` (int64 arrays), ``n_steps``, ``batch``, ``n``
+    and ``seed`` (or None)."""
+    with np.load(path, allow_pickle=False) as z:
+        if "format" not in z.files or str(z["format"]) != TAPE_FORMAT:
+            raise ValueError("%s is not a %s file" % (path, TAPE_FORMAT))
+        n_steps, batch, n = (int(x) for x in z["shape"])
+        return {"step": z["step"].astype(np.int64),
+                "neuron": z["neuron"].astype(np.int64),
+                "trial": z["trial"].astype(np.int64),
+                "n_steps": n_steps, "batch": batch, "n": n,
+                "seed": z["seed"].copy() if "seed" in z.files else None}
+
+
 class Tape:
     """Recorded Poisson outcomes: kicked neuron-trial indices per step."""
+
+    @classmethod
+    def load(cls, sim: DeviceSim, path):
+        """A tape file (:func:`save_tape`) on ``sim``'s device; its shape
+        must match the simulation's neurons and trials."""
+        z = load_tape(path)
+        if z["n"] != sim.n or z["batch"] != sim.batch:
+            raise ValueError("tape of %d trials x %d neurons, simulation of "
+                             "%d x %d" % (z["batch"], z["n"], sim.batch,
+                                          sim.n))
+        return cls(sim, z["step"], z["neuron"], z["trial"])
 
     def __init__(self, sim: DeviceSim, steps, neurons, trials=None):
         st = np.asarray(steps, dtype=np.int64).reshape(-1)

@@ -138,6 +138,33 @@ class Result:
         df["n_spikes"] = self.counts[idx]
         return df.sort_values("rate_hz", ascending=False)
 
+    def region_timeline(self):
+        """DataFrame region x bin of the mean firing rate in Hz
+        (sim.py:294-306): bins summed over each region's neurons, divided
+        by the region's size, the bin width and the trial count.  Regions
+        come from ``neurons["region"]`` where ``neurons`` is a FlyWire
+        neuron table; otherwise from the connectome's neuropil groups
+        (``group_<k>``)."""
+        import pandas as pd
+
+        nt = self.neurons
+        if isinstance(nt, pd.DataFrame) and "region" in nt.columns:
+            reg = nt["region"].to_numpy()
+        elif getattr(self, "groups", None) is not None:
+            reg = np.array(["group_%d" % g for g in self.groups])
+        else:
+            raise ValueError("no regions: neurons is not a table with a "
+                             "'region' column and the connectome has no "
+                             "groups")
+        codes, uniq = pd.factorize(reg)
+        n_reg = len(uniq)
+        acc = np.zeros((n_reg, self.n_bins), dtype=np.float64)
+        np.add.at(acc, codes, self.bins.T.astype(np.float64))
+        sizes = np.bincount(codes, minlength=n_reg).astype(np.float64)
+        bin_s = self.config.bin_ms / 1000.0
+        rates = acc / (sizes[:, None] * bin_s * self.config.n_trials)
+        return pd.DataFrame(rates, index=list(uniq))
+
     def meta(self, with_indices=False):
         return {
             "config": {k: v for k, v in asdict(self.config).items()
@@ -653,11 +680,9 @@ def build_engine(weights=None, neurons=None, silenced=None, params=None,
     (sim.py:956-987).  ``weights``: reference-format sparse COO W[post, pre],
     a ``{indices, values, shape}`` dict, or a Connectome."""
     if weights is None:
-        raise ValueError("no connectome given: the FlyWire tables are not part "
-                         "of this engine; pass weights, e.g. "
-                         "flybrain_amd.connectome.from_flywire_tables("
-                         "'2025_Connectivity_783.parquet', "
-                         "'2025_Completeness_783.csv')")
+        # the reference's default: its weight cache, else the FlyWire
+        # tables it reads (data.py:221-259); raises naming both paths
+        weights = C.load_default()
     if neurons is None:
         n = weights.n if isinstance(weights, C.Connectome) else \
             int((weights["shape"] if isinstance(weights, dict)
@@ -722,13 +747,14 @@ def run(stimuli, silenced=None, config: RunConfig | None = None, neurons=None,
     schedule change, with the drive read from a device table.
 
     ``tape`` (extension): (steps, neurons, trials) arrays of recorded
-    Bernoulli outcomes; when given they replace the Philox draw, so the run
-    can be checked against the reference fed the same tape.
+    Bernoulli outcomes, or the path of a tape file
+    (``device.save_tape``); when given they replace the Philox draw, so
+    the run can be checked against the reference fed the same tape.
     """
     config = config or RunConfig()
     silenced = np.asarray([] if silenced is None else silenced, dtype=np.int64)
     if weights is None:
-        raise ValueError("pass weights (a connectome)")
+        weights = C.load_default()  # as build_engine (data.py:221-259)
     if neurons is None:
         n = weights.n if isinstance(weights, C.Connectome) else \
             int((weights["shape"] if isinstance(weights, dict)
@@ -758,7 +784,12 @@ def run(stimuli, silenced=None, config: RunConfig | None = None, neurons=None,
     t_setup = time.perf_counter() - t0
     tape_drive = None
     if tape is not None:
-        tape_drive = tape if isinstance(tape, Tape) else Tape(sim, *tape)
+        if isinstance(tape, Tape):
+            tape_drive = tape
+        elif isinstance(tape, (str, os.PathLike)):
+            tape_drive = Tape.load(sim, tape)  # a save_tape file
+        else:
+            tape_drive = Tape(sim, *tape)
 
     rates = np.zeros(n, np.float32)  # identical across trials (sim.py:1062)
     counts = np.zeros(n, np.int64)
@@ -807,8 +838,32 @@ def run(stimuli, silenced=None, config: RunConfig | None = None, neurons=None,
         "windows": "up to %d steps per launch pair" % L.MAX_WINDOW,
         **engine.spmv_info(),
     }
-    return Result(counts, binned, config, list(stimuli), silenced, neurons,
-                  timing)
+    res = Result(counts, binned, config, list(stimuli), silenced, neurons,
+                 timing)
+    res.groups = engine._sim.net.conn.groups  # regions of region_timeline
+    return res
+
+
+STATE_FORMAT = "flybrain_amd.engine_state/1"
+
+
+def save_state(state, path):
+    """Write an :meth:`Engine.snapshot` dict as ``.npz`` (arrays only, no
+    pickled objects), like the reference's saved resume states
+    (lab/flybrain/replay.py:288-300)."""
+    arrays = {"format": np.array(STATE_FORMAT)}
+    arrays.update({k: np.asarray(v) for k, v in state.items()})
+    with open(path, "wb") as f:
+        np.savez_compressed(f, **arrays)
+
+
+def load_state(path):
+    """Read a snapshot written by :func:`save_state`; ready for
+    :meth:`Engine.restore`."""
+    with np.load(path, allow_pickle=False) as z:
+        if "format" not in z.files or str(z["format"]) != STATE_FORMAT:
+            raise ValueError("%s is not a %s file" % (path, STATE_FORMAT))
+        return {k: z[k].copy() for k in z.files if k != "format"}
 
 
 def _drive_for(sim, rates):

@@ -372,3 +372,128 @@ def test_network_without_edges_runs(road):
         ora.step_rates(r_np)
         assert np.array_equal(sp, ora.spikes), "step %d" % t
     assert ora.spikes.sum() >= 0 and sp[:, np.setdiff1d(np.arange(n), drv)].sum() == 0
+
+
+@pytest.mark.parametrize("road,batch", [("push", 3), ("pull", 64)])
+def test_recorded_philox_tape_replays_exactly(tmp_path, road, batch):
+    """The tape as a file: the kicks a Philox drive table made in a run
+    are recorded window by window, written as .npz, read back and fed to
+    a fresh simulation through Tape -- the raster and the final state are
+    those of the Philox run."""
+    from flybrain_amd.device import DeviceNet, DeviceSim, DriveTable, Tape
+    from flybrain_amd.device import save_tape
+
+    conn = C.synthetic(800, None, mean_out=50, count_exp=1.2, n_groups=4,
+                       seed=8)
+    net = DeviceNet(conn)
+    rng = np.random.default_rng(batch)
+    seeds = rng.integers(0, 2**63, size=batch, dtype=np.uint64)
+    ne = np.concatenate([rng.choice(conn.n, 30, replace=False)
+                         for _ in range(batch)])
+    tr = np.repeat(np.arange(batch), 30)
+
+    def make():
+        s = DeviceSim(net, batch, seeds=seeds, raster_len=200, road=road)
+        s.exempt_pairs(ne, tr)
+        return s
+
+    a = make()
+    table = DriveTable(a, ne, tr, tr, rng.uniform(200, 800, batch))
+    kicks = [a.record_window(20, table) for _ in range(10)]
+    st, nn, tt = (np.concatenate(x) for x in zip(*kicks))
+    assert st.size > 0 and st.max() < 200
+    path = tmp_path / "tape.npz"
+    save_tape(path, st, nn, tt, n=conn.n, batch=batch, seeds=seeds)
+    b = make()
+    tape = Tape.load(b, path)
+    for _ in range(10):
+        b.window(20, tape)
+    assert np.array_equal(a.read_raster(0, 200), b.read_raster(0, 200))
+    assert a.read_raster(0, 200).sum() > st.size  # recurrent spikes too
+    for name in ("v", "g", "st"):
+        assert torch.equal(getattr(a, name), getattr(b, name)), name
+
+
+def test_run_takes_a_tape_file(golden, tmp_path):
+    """run(tape=<path>) with the reference's recorded tape written as a
+    tape file gives the reference's run() result."""
+    from flybrain_amd.device import save_tape
+
+    z = golden("run_sched.npz")
+    conn = C.Connectome(int(z["n"]), z["conn_offsets"], z["conn_cols"],
+                        z["conn_vals"])
+    n, B = conn.n, int(z["batch"])
+    st = [S.Stimulus(indices=z["stim_a"], rate_hz=400.0, t_on_ms=5.0,
+                     t_off_ms=32.5, name="a"),
+          S.Stimulus(indices=z["stim_b"], rate_hz=900.0, t_on_ms=20.0,
+                     t_off_ms=None, name="b")]
+    bits = np.unpackbits(z["tape_bits"], axis=-1)[..., :n]
+    t, b, i = np.nonzero(bits)
+    save_tape(tmp_path / "tape.npz", t, i, b, n=n, batch=B)
+    cfg = S.RunConfig(t_run_s=float(z["t_run_s"]), bin_ms=float(z["bin_ms"]),
+                      n_trials=B, seed=0, device="cuda")
+    res = S.run(st, silenced=z["silenced"], config=cfg, weights=conn,
+                tape=str(tmp_path / "tape.npz"))
+    np.testing.assert_array_equal(res.counts, z["counts"])
+    np.testing.assert_array_equal(res.bins, z["bins"])
+
+
+@pytest.mark.parametrize("src_road,dst_road", [("push", "push"),
+                                               ("pull", "push"),
+                                               ("push", "pull")])
+def test_trial_state_file_resumes_in_another_slot(tmp_path, src_road,
+                                                  dst_road):
+    """A running trial saved (trial_state -> file) at a window boundary and
+    loaded into another slot of another simulation, later in that
+    simulation's clock and on either road, continues bit for bit as the
+    uninterrupted trial: raster of every later step and final state."""
+    from flybrain_amd.device import (DeviceNet, DeviceSim, DriveTable,
+                                     load_trial_state, save_trial_state)
+
+    conn = C.synthetic(900, None, mean_out=50, count_exp=1.2, n_groups=4,
+                       seed=12)
+    net = DeviceNet(conn)
+    rng = np.random.default_rng(5)
+    B = 64 if "pull" in (src_road, dst_road) else 4
+    seeds = rng.integers(0, 2**63, size=B, dtype=np.uint64)
+    drv = rng.choice(conn.n, 40, replace=False)
+    sil = rng.choice(conn.n, 6, replace=False)
+
+    def sim(road, seeds):
+        s = DeviceSim(net, B, seeds=seeds, raster_len=200, counts=True,
+                      road=road, trial_ids=np.zeros(B, np.uint32))
+        s.set_trial_end(np.arange(B), np.full(B, 10**9))
+        return s
+
+    def table(s, trials):
+        ne = np.tile(drv, len(trials))
+        tr = np.repeat(trials, drv.size)
+        return DriveTable(s, ne, tr, np.zeros(ne.size, np.int64),
+                          np.array([400.0], np.float32))
+
+    src = sim(src_road, seeds)
+    src.exempt_pairs(np.tile(drv, B), np.repeat(np.arange(B), drv.size))
+    src.set_flag_pairs(2, np.tile(sil, B), np.repeat(np.arange(B), sil.size))
+    ts = table(src, np.arange(B))
+    for _ in range(6):
+        src.window(20, ts)
+    k = 2  # the trial moved
+    save_trial_state(tmp_path / "trial.npz", src.trial_state(k))
+    ref_raster = []
+    for _ in range(8):
+        src.window(20, ts)
+        ref_raster.append(src.trial_raster(k, src.t - 20, src.t))
+    dst = sim(dst_road, np.zeros(B, np.uint64))
+    td = table(dst, np.array([B - 1]))
+    for _ in range(3):  # the destination's clock is elsewhere
+        dst.window(20, td)
+    dst.load_trial_state(B - 1, load_trial_state(tmp_path / "trial.npz"))
+    for j in range(8):
+        dst.window(20, td)
+        got = dst.trial_raster(B - 1, dst.t - 20, dst.t)
+        assert np.array_equal(got, ref_raster[j]), "window %d" % j
+    assert sum(r.sum() for r in ref_raster) > 0
+    for name in ("v", "g", "st", "counts"):
+        x = getattr(src, name).view(conn.n, src.bstride)[:, k]
+        y = getattr(dst, name).view(conn.n, dst.bstride)[:, B - 1]
+        assert torch.equal(x, y), name

@@ -8,6 +8,7 @@ import json
 
 import numpy as np
 import pandas as pd
+import pytest
 
 from flybrain_amd import sim as S
 
@@ -65,3 +66,94 @@ def test_rate_table_carries_neuron_table_columns():
     df = res.rate_table(0.5)
     assert list(df.columns) == cols + ["rate_hz", "n_spikes"]
     assert (df["root_id"] == ["root_id%d" % i for i in df.index]).all()
+
+
+def test_region_timeline_rates():
+    """Mean rate per region and bin (sim.py:294-306), from a neuron table
+    with a region column: neurons 0, 1 in 'AL', 2 in 'MB', 3 in 'AL';
+    bins of 10 ms, 2 trials.  Bin 0 of AL holds 2 + 4 + 0 spikes over 3
+    neurons: 6 / (3 * 0.01 s * 2) = 100 Hz."""
+    bins = np.array([[2, 4, 1, 0], [0, 0, 3, 6]], np.int16)
+    cfg = S.RunConfig(t_run_s=0.02, bin_ms=10.0, n_trials=2)
+    nt = pd.DataFrame({"region": ["AL", "AL", "MB", "AL"]})
+    res = S.Result(bins.sum(axis=0).astype(np.int64), bins, cfg, [],
+                   np.zeros(0, np.int64), nt, {})
+    tl = res.region_timeline()
+    assert list(tl.index) == ["AL", "MB"] and tl.shape == (2, 2)
+    np.testing.assert_allclose(tl.loc["AL"].to_numpy(), [100.0, 100.0])
+    np.testing.assert_allclose(tl.loc["MB"].to_numpy(), [50.0, 150.0])
+    # without a table, the connectome's neuropil groups name the regions
+    res2 = S.Result(res.counts, bins, cfg, [], np.zeros(0, np.int64),
+                    np.arange(4), {})
+    res2.groups = np.array([1, 1, 0, 1])
+    tl2 = res2.region_timeline()
+    np.testing.assert_allclose(tl2.loc["group_1"].to_numpy(), [100.0, 100.0])
+
+
+def test_state_and_tape_files_round_trip(tmp_path):
+    """Snapshot, tape and trial-state files are plain .npz (no pickles)
+    and read back exactly."""
+    import torch
+
+    from flybrain_amd import device as D
+
+    state = {"v": np.linspace(-60, -40, 12, dtype=np.float32).reshape(2, 6),
+             "ptr": np.int64(5), "gen": np.arange(16, dtype=np.uint8),
+             "calls": np.int64(105)}
+    S.save_state(state, tmp_path / "s.npz")
+    back = S.load_state(tmp_path / "s.npz")
+    assert set(back) == set(state)
+    for k in state:
+        np.testing.assert_array_equal(back[k], state[k])
+    assert int(back["ptr"]) == 5
+
+    steps = np.array([4, 0, 4, 9]), np.array([7, 1, 2, 7]), \
+        np.array([1, 0, 1, 0])
+    D.save_tape(tmp_path / "t.npz", *steps, n=10, batch=2,
+                seeds=np.array([3, 4], np.uint64))
+    z = D.load_tape(tmp_path / "t.npz")
+    assert (z["n_steps"], z["batch"], z["n"]) == (10, 2, 10)
+    got = sorted(zip(z["step"], z["neuron"], z["trial"]))
+    assert got == sorted(zip(*steps))
+    np.testing.assert_array_equal(z["seed"], [3, 4])
+    np.load(tmp_path / "t.npz", allow_pickle=False)  # nothing pickled
+    with pytest.raises(IndexError):
+        D.save_tape(tmp_path / "bad.npz", [0], [10], [0], n=10, batch=2)
+    with pytest.raises(ValueError):
+        D.load_tape(tmp_path / "s.npz")
+
+    ts = {"v": torch.arange(5, dtype=torch.float32), "g": torch.zeros(5),
+          "st": torch.full((5,), 22, dtype=torch.uint8),
+          "counts": torch.arange(5, dtype=torch.int32),
+          "inflight": torch.ones(20, 5, dtype=torch.int32),
+          "fl": np.zeros(5, np.uint8), "vhold": np.zeros(5, np.float32),
+          "steps_done": 40, "steps_left": 60, "seed": 2**63 + 5,
+          "trial_id": 0, "n": 5}
+    D.save_trial_state(tmp_path / "x.npz", ts)
+    tb = D.load_trial_state(tmp_path / "x.npz")
+    for k, v in ts.items():
+        if isinstance(v, torch.Tensor):
+            assert torch.equal(tb[k], v), k
+        elif isinstance(v, np.ndarray):
+            np.testing.assert_array_equal(tb[k], v)
+        else:
+            assert tb[k] == v, k
+
+
+def test_default_connectome_follows_the_reference_locations(tmp_path,
+                                                             monkeypatch):
+    """Without weights, the reference's cache/weights_coo.pt (under
+    $FLYBRAIN_ROOT) is loaded, safely (weights_only); with nothing there
+    the error names the paths."""
+    import torch
+
+    from flybrain_amd import connectome as C
+
+    monkeypatch.setenv("FLYBRAIN_ROOT", str(tmp_path))
+    with pytest.raises(FileNotFoundError, match="weights_coo.pt"):
+        C.load_default()
+    conn = C.synthetic(300, None, mean_out=10, n_groups=0, seed=4)
+    (tmp_path / "cache").mkdir()
+    torch.save(C.to_reference_coo(conn), tmp_path / "cache" / "weights_coo.pt")
+    got = C.load_default()
+    assert got.n == conn.n and got.fingerprint() == conn.fingerprint()
