@@ -160,11 +160,17 @@ def pull_window_bytes(sim, conn, W):
     act = sim.act.cpu().numpy().reshape(n, parts).view(np.uint64)
     pairs_src = popcount64(act & np.uint64(hmask)).sum(axis=1)
     pairs = int((conn.degree().astype(np.int64) * pairs_src).sum())
-    # run the window, then count the neuron-trials that sent in it
+    # run the window, then count the neuron-trials that sent in it (record
+    # words of a step are valid where the part's act bit for it is set)
     sim.pull(W)
     rec = sim.rec.cpu().numpy().reshape(n, L.REC_SLOTS, words).view(np.uint64)
-    slots = [u % L.REC_SLOTS for u in range(t0, t0 + W)]
-    sent = np.bitwise_or.reduce(rec[:, slots, :], axis=1)
+    act_after = sim.act.cpu().numpy().reshape(n, parts).view(np.uint64)
+    sent = np.zeros((n, words), np.uint64)
+    for u in range(t0, t0 + W):
+        j = u % L.REC_SLOTS
+        valid = ((act_after >> np.uint64(j)) & np.uint64(1)).astype(bool)
+        valid_w = np.repeat(valid, words // parts, axis=1)
+        sent |= np.where(valid_w, rec[:, j, :], np.uint64(0))
     spiking = int(popcount64(sent).sum())
     E = conn.n_edges
     parts_b = {
@@ -470,6 +476,7 @@ def leg_summary(rr):
             "graph_windows": rr["graph_windows"],
             "road": r.sim.road,
             "pull_tile": r.sim.pull_tile if r.sim.road == "pull" else None,
+            "gate": r.sim.gate,
             "autotuned": rr["tuned"]["times"] if rr["tuned"] else None,
             "spikes_per_trial_step": round(
                 rr["spikes"] / (rr["wl"].batch * rr["steps"]), 3),
@@ -617,6 +624,7 @@ def main():
                    "edges": conn.n_edges, "batch": B,
                    "road": hs.road,
                    "pull_tile": hs.pull_tile if hs.road == "pull" else None,
+                   "gate": hs.gate,
                    "autotuned": (head["tuned"]["times"] if head["tuned"]
                                  else None),
                    "window_steps": W,

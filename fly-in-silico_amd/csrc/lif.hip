@@ -1059,6 +1059,27 @@ void k_pull(fis_sim s, int W,
   const int64_t t0 = now(s);
   const int rbase = (int)((t0 + kRec - 20) % kRec);  // slot of step t0-20
   const int wbase = (int)(t0 % kRec);                // slot of step t0
+  // neuron at position p of the order (a null order is the identity)
+  auto neuron_at = [&](int p) { return s.order ? s.order[p] : p; };
+  // input gating: skip the in-edge walk when no input sent to the tile's
+  // neurons (in its trials) arrives at or after t0 (uniform over the tile)
+  bool gather = true;
+  if (s.due) {
+    int64_t latest = -1;
+#pragma unroll
+    for (int q = 0; q < P; ++q)
+      if (q < np) {
+        const int64_t d = s.due[(int64_t)neuron_at(p0 + q) * parts + part];
+        latest = d > latest ? d : latest;
+      }
+    gather = latest >= t0;
+  }
+  // this tile alone writes the act word of each of its neurons (for its
+  // part); read now, updated at the end with a plain store if it changes
+  // (other tiles read only the bits of the delay horizon, which the update
+  // keeps)
+  uint64_t act_old = 0;
+  if (tid < np) act_old = s.act[(int64_t)neuron_at(p0 + tid) * parts + part];
   const int pstride = kRec * words;                  // words per neuron
   const uint64_t wmask = (1ull << W) - 1;
 
@@ -1080,7 +1101,7 @@ void k_pull(fis_sim s, int W,
   const int tb = (w0 << 6) + (tid - pl * tpn);       // trial of this lane
   // uniform over a wave: a neuron of the tile, in a live 64-trial word
   const bool act = pl < np && (!s.word_end || t0 < s.word_end[tb >> 6]);
-  const int pre = pl < np ? s.order[p0 + pl] : 0;  // neuron of this lane
+  const int pre = pl < np ? neuron_at(p0 + pl) : 0;  // neuron of this lane
   const int idx = pre * bs + tb;
   const int wave = idx >> 6;
   const int nblk = (nb + kBlock - 1) / kBlock;  // raster rows: 256-lane groups
@@ -1094,7 +1115,7 @@ void k_pull(fis_sim s, int W,
     sb = s.sbits[idx];
   }
 
-  {  // 1. gather.  The tile's in-edges are read 64 at a time, one per lane,
+  if (gather) {  // 1. gather.  The tile's in-edges are read 64 at a time, one per lane,
      // with their presynaptic act words (the next 64 are fetched
      // meanwhile).  The active (in-edge, step) pairs of those 64 -- a step
      // of the horizon at which the presynaptic neuron spiked in some trial
@@ -1270,37 +1291,67 @@ void k_pull(fis_sim s, int W,
       for (uint32_t x = fired_bits; x; x &= x - 1u)
         atomicOr((uint64_t*)(mine + 4 * (__ffs(x) - 1)) + 1, me);
     __builtin_amdgcn_wave_barrier();
-    if (lane < W) {
-      const uint64_t word = cell[0];
-      if (word != 0ull) atomicOr(mine + 4 * WMAX, 1 << lane);
-      // `act` is uniform over a wave (n * bstride is a multiple of 64): the
-      // waves past the last neuron of the last tile store nothing
-      if (act) {
-        const int slot = wbase + lane < kRec ? wbase + lane : wbase + lane - kRec;
-        s.rec[(int64_t)pre * pstride + slot * words + (tb >> 6)] = word;
-        if (s.raster) {
-          int rrow = (int)(t0 % s.raster_len) + lane;  // raster_len >= 20
-          rrow = rrow >= s.raster_len ? rrow - s.raster_len : rrow;
-          s.raster[((int64_t)rrow * nblk + (wave >> 2)) * FIS_LINE_WORDS +
-                   (wave & 3)] = cell[1];
-        }
-      }
-    }
+    if (lane < W && cell[0] != 0ull) atomicOr(mine + 4 * WMAX, 1 << lane);
   }
   __syncthreads();
+
+  // The record words: step k's word of this wave is stored when the
+  // lane's neuron sent anything at step k in this part (its act bit is
+  // then set, and readers read every word of the part at that step), and
+  // always for the window's last step (an m_out re-send may set an act
+  // bit there later).  Other slots keep stale words that no reader reads:
+  // their act bits are clear.
+  uint32_t part_sent = 0;  // steps at which the lane's neuron sent (part)
+#pragma unroll
+  for (int j = 0; j < wpt; ++j)
+    part_sent |= (uint32_t)acc[(pl * tpn + j * kWave) * WMAX + 4 * WMAX];
+  if (lane < W && act) {
+    // `act` is uniform over a wave (n * bstride is a multiple of 64): the
+    // waves past the last neuron of the last tile store nothing
+    const uint64_t* cell =
+        (const uint64_t*)(acc + (tid & ~(kWave - 1)) * WMAX + 4 * lane);
+    if (((part_sent >> lane) & 1u) || lane == W - 1) {
+      const int slot = wbase + lane < kRec ? wbase + lane : wbase + lane - kRec;
+      s.rec[(int64_t)pre * pstride + slot * words + (tb >> 6)] = cell[0];
+    }
+    if (s.raster) {
+      int rrow = (int)(t0 % s.raster_len) + lane;  // raster_len >= 20
+      rrow = rrow >= s.raster_len ? rrow - s.raster_len : rrow;
+      s.raster[((int64_t)rrow * nblk + (wave >> 2)) * FIS_LINE_WORDS +
+               (wave & 3)] = cell[1];
+    }
+  }
 
   // 3. act bits of the window's steps: clear, then set
   if (tid < np) {
     uint32_t bits32 = 0;
     for (int q = 0; q < wpt; ++q)
       bits32 |= (uint32_t)acc[(tid * tpn + q * kWave) * WMAX + 4 * WMAX];
-    uint64_t* a = s.act + (int64_t)s.order[p0 + tid] * parts + part;
-    __hip_atomic_fetch_and(a, ~rot40(wmask, wbase), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    const uint64_t bits = rot40((uint64_t)bits32, wbase);
-    if (bits)
-      __hip_atomic_fetch_or(a, bits, __ATOMIC_RELAXED,
-                            __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t nw = (act_old & ~rot40(wmask, wbase)) |
+                        rot40((uint64_t)bits32, wbase);
+    if (nw != act_old)
+      s.act[(int64_t)neuron_at(p0 + tid) * parts + part] = nw;
+  }
+
+  // 4. input gating: a neuron of the tile that sent anything (in this
+  // part) stamps its targets with the last step its spikes can arrive at.
+  // Every workgroup of a launch stores the same stamp, and stamps only
+  // grow from launch to launch, so plain stores suffice.
+  if (s.due) {
+    const int64_t stamp = t0 + W - 1 + kDelay;
+    for (int q = 0; q < np; ++q) {
+      uint32_t sent = 0;
+#pragma unroll
+      for (int j = 0; j < wpt; ++j)
+        sent |= (uint32_t)acc[(q * tpn + j * kWave) * WMAX + 4 * WMAX];
+      if (!sent) continue;  // uniform: every thread reads the same cells
+      const int src = neuron_at(p0 + q);
+      const int64_t b1 = s.offsets[src + 1];
+      for (int64_t k = s.offsets[src] + tid; k < b1; k += TILE) {
+        int64_t* d = s.due + (int64_t)s.cols[k] * parts + part;
+        if (*d < stamp) *d = stamp;  // a target stamped already is not rewritten
+      }
+    }
   }
 }
 
@@ -1330,10 +1381,18 @@ __global__ __launch_bounds__(kBlock) void k_pull_resend(fis_sim s,
   } else {
     __hip_atomic_fetch_or(w, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int tile = pull_tile_of(s);
-    __hip_atomic_fetch_or(s.act + (int64_t)pre * pull_parts(s.bstride, tile) +
-                              b / pull_trials_per_part(s.bstride, tile),
-                          1ull << slot, __ATOMIC_RELAXED,
-                          __HIP_MEMORY_SCOPE_AGENT);
+    const int parts = pull_parts(s.bstride, tile);
+    const int part = b / pull_trials_per_part(s.bstride, tile);
+    __hip_atomic_fetch_or(s.act + (int64_t)pre * parts + part, 1ull << slot,
+                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (s.due) {  // the spike of step now - 1 now reaches its targets
+      const int64_t stamp = now(s) - 1 + kDelay;
+      for (int64_t k = s.offsets[pre]; k < s.offsets[pre + 1]; ++k)
+        __hip_atomic_fetch_max((unsigned long long*)(s.due +
+                                   (int64_t)s.cols[k] * parts + part),
+                               (unsigned long long)stamp, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 
@@ -1498,8 +1557,7 @@ int check_pull(const fis_sim* s) {
   const int bs = s->bstride;
   if (bs < kWave || bs > 1024 || (bs & (bs - 1)) != 0)
     return (int)hipErrorInvalidValue;
-  if (!s->in_offsets || !s->in_cols || !s->in_vals || !s->rec || !s->act ||
-      !s->order)
+  if (!s->in_offsets || !s->in_cols || !s->in_vals || !s->rec || !s->act)
     return (int)hipErrorInvalidValue;
   if (s->pull_tile != 0 && s->pull_tile != 128 && s->pull_tile != 256 &&
       s->pull_tile != 512)
@@ -1592,6 +1650,7 @@ int64_t fis_struct_layout(int what) {
     case 9: return (int64_t)offsetof(fis_sim, pull_tile);
     case 10: return (int64_t)offsetof(fis_sim, clock_ahead);
     case 11: return (int64_t)offsetof(fis_sim, drive_dense);
+    case 12: return (int64_t)offsetof(fis_sim, due);
     default: return -1;
   }
 }

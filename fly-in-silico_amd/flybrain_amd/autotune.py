@@ -3,7 +3,9 @@
 A batch of trials runs on one of two roads (push: event scatter into an
 int32 ring; pull: a fused gather + update per window, 64..1024 trials) and,
 on the pull road, with one of the compiled workgroup shapes
-(``_lib.PULL_TILES`` neuron-trials per workgroup).  Which is fastest
+(``_lib.PULL_TILES`` neuron-trials per workgroup) and with or without input
+gating (tiles no input reaches skip their in-edge walk, at the price of
+stamping every sender's targets).  Which is fastest
 depends on the batch, the connectome and how active it is, so
 :func:`tune` times every candidate on the workload itself -- a short
 warm-up to reach its working activity, then a few eager windows bracketed
@@ -31,11 +33,17 @@ def cache_path():
 
 
 def candidates(batch):
-    """(road, pull_tile) pairs that can run ``batch`` trials."""
-    out = [("push", None)]
+    """(road, pull_tile, gate) choices that can run ``batch`` trials: the
+    push road, and the pull road with each compiled workgroup shape,
+    with and without input gating."""
+    out = [("push", None, False)]
     if PULL_MIN_BATCH <= batch <= L.PULL_MAX:
-        out += [("pull", t) for t in L.PULL_TILES]
+        out += [("pull", t, g) for g in (False, True) for t in L.PULL_TILES]
     return out
+
+
+def _label(road, tile, gate):
+    return "%s/%s%s" % (road, tile or "-", "/gated" if gate else "")
 
 
 def library_fingerprint():
@@ -96,14 +104,16 @@ def _store(path, table):
         pass
 
 
-def time_candidate(net, wl, road, pull_tile, warmup_steps=200, windows=10):
+def time_candidate(net, wl, road, pull_tile, gate=False, warmup_steps=200,
+                   windows=10):
     """Mean wall time of one 20-step window (ms) of ``wl`` on (road,
-    pull_tile), eager launches, after ``warmup_steps`` steps."""
+    pull_tile, gate), eager launches, after ``warmup_steps`` steps."""
     import torch
 
     from .batch import BatchRunner
 
-    r = BatchRunner(net, wl, counts=False, road=road, pull_tile=pull_tile)
+    r = BatchRunner(net, wl, counts=False, road=road, pull_tile=pull_tile,
+                    gate=gate)
     r.advance(warmup_steps, use_graph=False)
     stream = torch.cuda.current_stream()
     e0 = torch.cuda.Event(enable_timing=True)
@@ -120,24 +130,26 @@ def time_candidate(net, wl, road, pull_tile, warmup_steps=200, windows=10):
 def tune(net, wl, cache=True, timer=None, **kw):
     """Pick the fastest (road, pull_tile) for ``wl`` on ``net``.
 
-    Returns ``{"best": {"road", "pull_tile", "ms_per_window"}, "times":
-    {"road/tile": ms}}``.  ``timer(net, wl, road, tile) -> ms`` replaces
-    the device measurement (tests); ``kw`` goes to :func:`time_candidate`.
+    Returns ``{"best": {"road", "pull_tile", "gate", "ms_per_window"},
+    "times": {"road/tile[/gated]": ms}}``.  ``timer(net, wl, road, tile,
+    gate) -> ms`` replaces the device measurement (tests); ``kw`` goes to
+    :func:`time_candidate`.
     """
     path = cache_path()
     key = cache_key(net, wl)
     table = _load(path) if cache else {}
     if key in table:
         return table[key]
-    timer = timer or (lambda n, w, road, tile: time_candidate(n, w, road,
-                                                               tile, **kw))
+    timer = timer or (lambda n, w, road, tile, gate: time_candidate(
+        n, w, road, tile, gate, **kw))
     times = {}
     best = None
-    for road, tile in candidates(wl.batch):
-        ms = float(timer(net, wl, road, tile))
-        times["%s/%s" % (road, tile or "-")] = round(ms, 4)
+    for road, tile, gate in candidates(wl.batch):
+        ms = float(timer(net, wl, road, tile, gate))
+        times[_label(road, tile, gate)] = round(ms, 4)
         if best is None or ms < best["ms_per_window"]:
-            best = {"road": road, "pull_tile": tile, "ms_per_window": ms}
+            best = {"road": road, "pull_tile": tile, "gate": gate,
+                    "ms_per_window": ms}
     result = {"best": best, "times": times}
     if cache:
         table[key] = result

@@ -28,22 +28,25 @@ from .workloads import Workload
 class BatchRunner:
     def __init__(self, net: DeviceNet, wl: Workload, params=None,
                  raster_len=0, counts=True, pop_counts=False, window=20,
-                 road="auto", fused_drive=True, pull_tile=None):
-        """``road="tuned"`` takes the road and pull-road workgroup shape that
-        :func:`flybrain_amd.autotune.tune` measured fastest for this
-        connectome, batch and workload (timed once, then cached)."""
+                 road="auto", fused_drive=True, pull_tile=None, gate=False):
+        """``road="tuned"`` takes the road, pull-road workgroup shape and
+        input gating that :func:`flybrain_amd.autotune.tune` measured
+        fastest for this connectome, batch and workload (timed once, then
+        cached)."""
         if not 1 <= window <= L.MAX_WINDOW:
             raise ValueError("window must be 1..%d" % L.MAX_WINDOW)
         if road == "tuned":
             from .autotune import tune
             best = tune(net, wl)["best"]
             road, pull_tile = best["road"], best["pull_tile"]
+            gate = bool(best.get("gate", False))
         self.wl = wl
         self.window = int(window)
         sim = DeviceSim(net, wl.batch, params=params, seeds=wl.seeds,
                         trial_ids=np.zeros(wl.batch, np.uint32),
                         counts=counts, pop_counts=pop_counts,
-                        raster_len=raster_len, road=road, pull_tile=pull_tile)
+                        raster_len=raster_len, road=road, pull_tile=pull_tile,
+                        gate=gate)
         self.sim = sim
         # per-trial refractory exemption of driven neurons; the counter
         # starts at refrac_steps = 0 for them (sim.py:371-388)
@@ -82,7 +85,8 @@ class BatchRunner:
         """
         s = self.sim
         names = ["v", "g", "st", "ring", "sbits", "ev_count", "clock"]
-        names += [k for k in ("counts", "pop_counts", "raster", "rec", "act")
+        names += [k for k in ("counts", "pop_counts", "raster", "rec", "act",
+                              "due")
                   if getattr(s, k) is not None]
         saved = {k: getattr(s, k).clone() for k in names}
         t0 = s.t

@@ -89,7 +89,12 @@ class DeviceNet:
         self.vals = torch.from_numpy(_nonempty(conn.vals)).to(self.device)
         self._layouts = {}
 
-    locality = True  # pull road: order neurons by group (see pull_layout)
+    # pull road: order neurons by neuropil group (see pull_layout).  Off by
+    # default: measured on MI355X at 138,639 neurons, the identity order
+    # (no dependent load of each tile's neuron numbers) was faster on every
+    # workload (B = 256 sensory 0.902 -> 0.882 ms per window, B = 64
+    # 0.179 -> 0.176, plain B = 256 gated 0.312 -> 0.281)
+    locality = False
 
     def pull_layout(self, bstride):
         """The pull road's neuron order and its in-edge CSR by position.
@@ -147,7 +152,10 @@ class DeviceSim:
     def __init__(self, net: DeviceNet, batch: int, params=None, seeds=None,
                  trial_ids=None, no_refractory=None, spikes_out=False,
                  counts=False, pop_counts=False, raster_len=0, road="push",
-                 pull_tile=None):
+                 pull_tile=None, gate=False):
+        """``gate`` (pull road): skip the in-edge walk of tiles that no
+        input can reach in a window (fis_sim.due); fixed for the
+        simulation's life, chosen by measurement (flybrain_amd.autotune)."""
         self.lib = L.lib()
         self.road = pick_road(int(batch), road)
         # pull road: neuron-trials per workgroup (fixed for the simulation:
@@ -217,15 +225,22 @@ class DeviceSim:
                                    * L.LINE_WORDS, dtype=torch.int64, **z)
                        if raster_len else None)
         if self.road == "pull":
-            self.order, self.in_offsets, self.in_cols, self.in_vals = \
+            order, self.in_offsets, self.in_cols, self.in_vals = \
                 net.pull_layout(bs)
+            # the identity order goes to the kernel as a null pointer: no
+            # dependent load of each tile's neuron numbers
+            identity = not (net.locality and net.conn.groups is not None)
+            self.order = None if identity else order
             self.rec = torch.zeros(L.REC_SLOTS * n * (bs // 64),
                                    dtype=torch.int64, **z)
             self.act = torch.zeros(n * max(1, bs // self.pull_tile),
                                    dtype=torch.int64, **z)
+            self.due = (torch.zeros(n * max(1, bs // self.pull_tile),
+                                    dtype=torch.int64, **z) if gate else None)
         else:
             self.order = self.in_offsets = self.in_cols = self.in_vals = None
-            self.rec = self.act = None
+            self.rec = self.act = self.due = None
+        self.gate = self.due is not None
         self.t = 0  # host mirror of the device clock
         self._open_window = 0
         self.ring_until = 0  # pull road: steps < this also read the ring
@@ -270,7 +285,8 @@ class DeviceSim:
             setattr(s, name, t.data_ptr() if t is not None else None)
         s.raster_len = self.raster_len
         for name in ("in_offsets", "in_cols", "in_vals", "rec", "act",
-                     "order", "word_end", "drive_group", "drive_rate"):
+                     "order", "word_end", "drive_group", "drive_rate",
+                     "due"):
             t = getattr(self, name, None)
             setattr(s, name, t.data_ptr() if t is not None else None)
         dims = getattr(self, "_drive_dims", (0, 0, 0))

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define FIS_ABI_VERSION 11
+#define FIS_ABI_VERSION 12
 #define FIS_MAX_WINDOW 20   /* steps per window: delay horizon 19 + 1     */
 #define FIS_RING_LEN 40     /* int32 ring slots, two delays: the spikes of
                                step u are summed into slot (u + 21) mod 40,
@@ -192,6 +192,14 @@ typedef struct {
    * would set, without its launch (Engine.step's rates, sim.py:745-771).
    * Nullable; not together with drive_group; fis_pull_window refuses it. */
   const float* drive_dense;
+  /* Pull road input gating (nullable = off): [n][max(1, bstride /
+   * pull_tile)] int64, per neuron and tile part the latest step at which
+   * input sent to it may arrive.  A window that sends a spike of a part
+   * sets its targets' entries (W^T CSR) to the window's last step + 20; a
+   * window whose tile's entries are all below its first step has no
+   * synaptic input and skips the in-edge walk.  Zero at the start; fixed
+   * for a simulation's life (on from its first window, or never). */
+  int64_t* due;
 } fis_sim;
 
 /* Drive table entry: neuron-trial `idx` is driven at rate
@@ -346,8 +354,9 @@ int fis_flag_pairs(const fis_sim* s, const int32_t* idx, int n,
 
 /* sizeof / offsetof of the structs above (0: sizeof fis_sim, 1: offsets,
  * 2: w_scale, 3: refrac_steps, 4: raster_len, 5: sizeof fis_event,
- * 6: sizeof fis_drive_entry, 7: in_offsets, 8: t_origin), so a binding
- * can check its mirror. */
+ * 6: sizeof fis_drive_entry, 7: in_offsets, 8: t_origin, 9: pull_tile,
+ * 10: clock_ahead, 11: drive_dense, 12: due), so a binding can check its
+ * mirror. */
 int64_t fis_struct_layout(int what);
 
 /* Number of update waves (event segments) for a given n and bstride. */

@@ -21,28 +21,34 @@ def fake(batch, fp="c0ffee", seed=0):
 
 
 def test_candidates_follow_the_roads():
-    assert T.candidates(1) == [("push", None)]
-    assert T.candidates(63) == [("push", None)]
+    assert T.candidates(1) == [("push", None, False)]
+    assert T.candidates(63) == [("push", None, False)]
     got = T.candidates(256)
-    assert got[0] == ("push", None)
-    assert [t for r, t in got if r == "pull"] == list(L.PULL_TILES)
-    assert T.candidates(L.PULL_MAX + 1) == [("push", None)]
+    assert got[0] == ("push", None, False)
+    pull = [(t, g) for r, t, g in got if r == "pull"]
+    assert sorted(pull) == sorted((t, g) for t in L.PULL_TILES
+                                  for g in (False, True))
+    assert T.candidates(L.PULL_MAX + 1) == [("push", None, False)]
 
 
 def test_tune_picks_the_fastest_and_caches(tmp_path, monkeypatch):
     monkeypatch.setenv("FLYBRAIN_TUNE_CACHE", str(tmp_path / "t.json"))
     net, wl = fake(256)
-    cost = {("push", None): 8.0, ("pull", 128): 1.2, ("pull", 256): 1.1,
-            ("pull", 512): 1.4}
+    cost = {("push", None, False): 8.0, ("pull", 128, False): 1.2,
+            ("pull", 256, False): 1.1, ("pull", 512, False): 1.4,
+            ("pull", 128, True): 1.3, ("pull", 256, True): 0.9,
+            ("pull", 512, True): 1.5}
     calls = []
 
-    def timer(n, w, road, tile):
-        calls.append((road, tile))
-        return cost[(road, tile)]
+    def timer(n, w, road, tile, gate):
+        calls.append((road, tile, gate))
+        return cost[(road, tile, gate)]
 
     res = T.tune(net, wl, timer=timer)
     assert res["best"]["road"] == "pull" and res["best"]["pull_tile"] == 256
-    assert res["times"]["push/-"] == 8.0 and len(res["times"]) == 4
+    assert res["best"]["gate"] is True
+    assert res["times"]["push/-"] == 8.0 and len(res["times"]) == 7
+    assert res["times"]["pull/256/gated"] == 0.9
     assert sorted(calls, key=str) == sorted(cost, key=str)
     # cached: no second measurement, same answer
     again = T.tune(net, wl, timer=lambda *a: 1 / 0)
@@ -51,7 +57,7 @@ def test_tune_picks_the_fastest_and_caches(tmp_path, monkeypatch):
     assert list(table) == [T.cache_key(net, wl)]
     # another connectome or batch is measured anew
     net2, _ = fake(256, fp="beef")
-    assert T.tune(net2, wl, timer=lambda n, w, r, t: 2.0 if r == "push"
+    assert T.tune(net2, wl, timer=lambda n, w, r, t, g: 2.0 if r == "push"
                   else 3.0)["best"]["road"] == "push"
 
 
@@ -61,7 +67,7 @@ def test_unwritable_cache_is_skipped(tmp_path, monkeypatch):
     monkeypatch.setenv("FLYBRAIN_TUNE_CACHE", str(blocker / "sub" / "t.json"))
     net, wl = fake(32)
     res = T.tune(net, wl, timer=lambda *a: 5.0)
-    assert res["best"] == {"road": "push", "pull_tile": None,
+    assert res["best"] == {"road": "push", "pull_tile": None, "gate": False,
                            "ms_per_window": 5.0}
 
 

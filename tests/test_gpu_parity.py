@@ -175,17 +175,37 @@ def test_refractory_periods_match_oracle(batch, road, refrac):
                    params=dict(O.MODEL_PARAMS, tRefrac=refrac * 0.1))
 
 
-def philox_batched(batch, road, fused, tile=None, params=None):
-    conn = C.synthetic(512, None, mean_out=40, count_exp=1.2, n_groups=4,
-                       seed=21)
+@pytest.mark.parametrize("batch,tile,n_drive", [(64, 256, (1, 3)),
+                                                (256, 256, (1, 3)),
+                                                (100, 512, (1, 4)),
+                                                (600, 128, (1, 2)),
+                                                (256, 128, (5, 40))])
+def test_pull_gated_matches_oracle(batch, tile, n_drive):
+    """Input gating (tiles that no input can reach skip their in-edge
+    walk): with a sparse drive most tiles skip most windows, and raster
+    and state still equal the oracle; so with a dense one."""
+    sim = philox_batched(batch, "pull", False, tile, gate=True,
+                         n_drive=n_drive, conn_kw=dict(mean_out=6))
+    due = sim.due.cpu().numpy()
+    assert (due >= sim.t).any()
+    if n_drive[1] <= 4:  # sparse: some tiles had no input to gather
+        assert (due < sim.t).any()
+
+
+def philox_batched(batch, road, fused, tile=None, params=None, gate=False,
+                   n_drive=(5, 40), conn_kw=None):
+    kw = dict(mean_out=40, count_exp=1.2, n_groups=4, seed=21)
+    kw.update(conn_kw or {})
+    conn = C.synthetic(512, None, **kw)
     n = conn.n
     rng = np.random.default_rng(batch)
     seeds = rng.integers(0, 2**63, size=batch, dtype=np.uint64)
-    drv = [rng.choice(n, rng.integers(5, 40), replace=False) for _ in range(batch)]
+    drv = [rng.choice(n, rng.integers(*n_drive), replace=False)
+           for _ in range(batch)]
     rate = [float(rng.uniform(50, 900)) for _ in range(batch)]
     net = DeviceNet(conn)
     sim = DeviceSim(net, batch, params=params, seeds=seeds, raster_len=40,
-                    road=road, pull_tile=tile)
+                    road=road, pull_tile=tile, gate=gate)
     if params:
         assert sim.c["refrac_steps"] == round(params["tRefrac"] / 0.1)
     ne = np.concatenate(drv)
@@ -208,6 +228,7 @@ def philox_batched(batch, road, fused, tile=None, params=None):
             assert np.array_equal(raster[k], ora.spikes.astype(bool)), \
                 "raster differs at step %d" % (a + k)
     compare_state(sim, ora, "end")
+    return sim
 
 
 @pytest.mark.parametrize("batch", [64, 100, 200, 600])
@@ -215,7 +236,9 @@ def test_pull_road_uneven_windows_and_out_masks(batch):
     """Pull road with a partial last tile (509 neurons), windows of 1..20
     steps (records of one delay horizon span several windows) and m_out
     switched between windows on spiking neurons: raster, state and the
-    in-flight input equal the oracle; the push road gives the same."""
+    in-flight input equal the oracle, with and without input gating (the
+    m_out re-send stamps the targets it re-enables); the push road gives
+    the same."""
     conn = C.synthetic(509, None, mean_out=45, count_exp=1.15, n_groups=4,
                        seed=23)
     n = conn.n
@@ -228,8 +251,9 @@ def test_pull_road_uneven_windows_and_out_masks(batch):
     tr = np.concatenate([np.full(len(d), b) for b, d in enumerate(drv)])
     net = DeviceNet(conn)
     sims = {}
-    for road in ("pull", "push"):
-        sim = DeviceSim(net, batch, seeds=seeds, raster_len=40, road=road)
+    for road in ("pull", "pull_gated", "push"):
+        sim = DeviceSim(net, batch, seeds=seeds, raster_len=40,
+                        road=road.split("_")[0], gate=road == "pull_gated")
         sim.exempt_pairs(ne, tr)
         sims[road] = (sim, DriveTable(sim, ne, tr, tr, rate))
     ora = O.OracleEngine(conn, batch, seeds=seeds, no_refractory_trials=drv)
@@ -266,10 +290,11 @@ def test_pull_road_uneven_windows_and_out_masks(batch):
         for r, (s, _) in sims.items():
             compare_state(s, ora, "%s road, step %d" % (r, t))
         img = {r: s.ring_image().cpu().numpy() for r, (s, _) in sims.items()}
-        np.testing.assert_array_equal(
-            img["pull"].reshape(20, n, -1)[:, :, :batch],
-            img["push"].reshape(20, n, -1)[:, :, :batch],
-            err_msg="input in flight at step %d" % t)
+        for r in ("pull", "pull_gated"):
+            np.testing.assert_array_equal(
+                img[r].reshape(20, n, -1)[:, :, :batch],
+                img["push"].reshape(20, n, -1)[:, :, :batch],
+                err_msg="%s: input in flight at step %d" % (r, t))
     assert ora.spikes.any()
 
 
