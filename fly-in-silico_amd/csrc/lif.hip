@@ -157,6 +157,19 @@ __device__ __forceinline__ int n_waves_of(const fis_sim& s) {
   return (s.n * s.bstride + kWave - 1) >> 6;
 }
 
+// Pull road: a kick, a reset or a flag change of neuron-trial idx ends the
+// calm of its (neuron, tile part), which would otherwise let the next
+// window skip it unread.  Every writer stores the same zero.
+__device__ __forceinline__ void unsettle(const fis_sim& s, int64_t idx) {
+  if (!s.calm) return;
+  const int tile = s.pull_tile ? s.pull_tile : FIS_PULL_TILE;
+  const int tpn = s.bstride < tile ? s.bstride : tile;
+  const int64_t i = idx / s.bstride;
+  const int b = (int)(idx - i * s.bstride);
+  uint8_t* c = s.calm + i * (s.bstride / tpn) + b / tpn;
+  if (*c) *c = 0;
+}
+
 // ------------------------------------------------------------- drive
 
 __global__ __launch_bounds__(kBlock) void k_drive_rates(fis_sim s,
@@ -176,6 +189,7 @@ __global__ __launch_bounds__(kBlock) void k_drive_rates(fis_sim s,
                                          s.seeds[b], [p]() { return p; });
   }
   s.sbits[idx] = bits;
+  if (bits) unsettle(s, idx);
 }
 
 // Eight waves per SIMD: the compiler's choice (66 VGPRs) left room for
@@ -200,7 +214,10 @@ void k_drive_table(
                 t));
   // entries of neighbouring neuron-trials may sit in other workgroups: set
   // the bits with an atomic so no two workgroups store into one line
-  if (bits) atomicOr(&s.sbits[e.idx], bits);
+  if (bits) {
+    atomicOr(&s.sbits[e.idx], bits);
+    unsettle(s, e.idx);
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_drive_tape(fis_sim s,
@@ -212,8 +229,10 @@ __global__ __launch_bounds__(kBlock) void k_drive_tape(fis_sim s,
   if (t >= n_tape_steps) return;
   const int64_t beg = tape_ptr[t], end = tape_ptr[t + 1];
   for (int64_t e = beg + (int64_t)blockIdx.x * kBlock + threadIdx.x; e < end;
-       e += (int64_t)gridDim.x * kBlock)
+       e += (int64_t)gridDim.x * kBlock) {
     atomicOr(&s.sbits[tape_idx[e]], 1u << blockIdx.y);
+    unsettle(s, tape_idx[e]);
+  }
 }
 
 // ------------------------------------------------------------- update
@@ -1028,7 +1047,7 @@ __host__ __device__ inline int pull_parts(int bs, int tile) {
 // kernel is resident one fewer per CU than the compiler's occupancy
 // estimate whenever the kernel uses more than ~80 (96 SGPRs: 7 of 8
 // workgroups, 1.465 ms per window; capped at 80: 8, 1.417 ms).
-template <int WMAX, int BS, int TILE>
+template <int WMAX, int BS, int TILE, bool GATE>
 __global__ __launch_bounds__(TILE) __attribute__((amdgpu_num_sgpr(80)))
 __attribute__((amdgpu_waves_per_eu(8, 8)))
 void k_pull(fis_sim s, int W,
@@ -1059,12 +1078,14 @@ void k_pull(fis_sim s, int W,
   const int64_t t0 = now(s);
   const int rbase = (int)((t0 + kRec - 20) % kRec);  // slot of step t0-20
   const int wbase = (int)(t0 % kRec);                // slot of step t0
+  const int pstride = kRec * words;                  // words per neuron
+  const uint64_t wmask = (1ull << W) - 1;
   // neuron at position p of the order (a null order is the identity)
   auto neuron_at = [&](int p) { return s.order ? s.order[p] : p; };
   // input gating: skip the in-edge walk when no input sent to the tile's
   // neurons (in its trials) arrives at or after t0 (uniform over the tile)
   bool gather = true;
-  if (s.due) {
+  if (GATE) {
     int64_t latest = -1;
 #pragma unroll
     for (int q = 0; q < P; ++q)
@@ -1080,8 +1101,48 @@ void k_pull(fis_sim s, int W,
   // keeps)
   uint64_t act_old = 0;
   if (tid < np) act_old = s.act[(int64_t)neuron_at(p0 + tid) * parts + part];
-  const int pstride = kRec * words;                  // words per neuron
-  const uint64_t wmask = (1ull << W) - 1;
+  // calm: every neuron of the tile sat at a fixed point through the last
+  // window, and no input, kick or carried-over ring input reaches it in
+  // this one -- the window changes nothing but the record and act words,
+  // and its state is not read (uniform over the tile)
+  bool calm = false;
+  uint32_t calm_old = 0;  // bit q: neuron q of the tile was calm
+  if (GATE && s.calm) {
+#pragma unroll
+    for (int q = 0; q < P; ++q)
+      if (q < np && s.calm[(int64_t)neuron_at(p0 + q) * parts + part])
+        calm_old |= 1u << q;
+    calm = !gather && ring_steps == 0 && calm_old == (1u << np) - 1u;
+  }
+  if (calm) {
+    const int lane = tid & (kWave - 1);
+    const int pl = tid / tpn;
+    const int tb = (w0 << 6) + (tid - pl * tpn);
+    const bool live = pl < np && (!s.word_end || t0 < s.word_end[tb >> 6]);
+    if (live && lane < W) {
+      const int pre = neuron_at(p0 + pl);
+      if (lane == W - 1) {  // the slot an m_out re-send may reach
+        const int slot =
+            wbase + lane < kRec ? wbase + lane : wbase + lane - kRec;
+        s.rec[(int64_t)pre * pstride + slot * words + (tb >> 6)] = 0ull;
+      }
+      if (s.raster) {
+        const int nblk = (s.n * bs + kBlock - 1) / kBlock;
+        const int wave = (pre * bs + tb) >> 6;
+        int rrow = (int)(t0 % s.raster_len) + lane;
+        rrow = rrow >= s.raster_len ? rrow - s.raster_len : rrow;
+        s.raster[((int64_t)rrow * nblk + (wave >> 2)) * FIS_LINE_WORDS +
+                 (wave & 3)] = 0ull;
+      }
+    }
+    if (tid < np) {
+      const uint64_t nw = act_old & ~rot40(wmask, wbase);
+      if (nw != act_old)
+        s.act[(int64_t)neuron_at(p0 + tid) * parts + part] = nw;
+    }
+    return;  // whole workgroup, before any barrier
+  }
+
 
 #pragma unroll
   for (int j = 0; j < WMAX; j += 4)
@@ -1268,6 +1329,11 @@ void k_pull(fis_sim s, int W,
     else
       store_cell(s, c, idx, pre, nspk);
   }
+  // calm after this window: quiet with the refractory counter saturated
+  // (the next quiet window stores nothing then); kept in the wave's cell
+  // 4 WMAX + 1 below
+  const bool wave_calm = GATE && s.calm && quiet && act &&
+                         !any_lane(c.r < (int)FIS_ST_RMAX);
 
   // The wave's record words of the window (and raster rows): lane k < W
   // owns step k's.  This wave's 64 x WMAX cells are its own and already
@@ -1283,6 +1349,7 @@ void k_pull(fis_sim s, int W,
       cell[1] = 0ull;
     }
     if (lane == W) mine[4 * WMAX] = 0;
+    if (lane == 0) mine[4 * WMAX + 1] = wave_calm ? 1 : 0;
     __builtin_amdgcn_wave_barrier();
     const uint64_t me = 1ull << lane;
     for (uint32_t x = sent; x; x &= x - 1u)
@@ -1331,13 +1398,21 @@ void k_pull(fis_sim s, int W,
                         rot40((uint64_t)bits32, wbase);
     if (nw != act_old)
       s.act[(int64_t)neuron_at(p0 + tid) * parts + part] = nw;
+    if (GATE && s.calm) {  // every wave of the neuron's part ended calm
+      uint32_t now_calm = 1u;
+      for (int q = 0; q < wpt; ++q)
+        now_calm &= (uint32_t)acc[(tid * tpn + q * kWave) * WMAX + 4 * WMAX + 1];
+      if (now_calm != ((calm_old >> tid) & 1u))
+        s.calm[(int64_t)neuron_at(p0 + tid) * parts + part] =
+            (uint8_t)now_calm;
+    }
   }
 
   // 4. input gating: a neuron of the tile that sent anything (in this
   // part) stamps its targets with the last step its spikes can arrive at.
   // Every workgroup of a launch stores the same stamp, and stamps only
   // grow from launch to launch, so plain stores suffice.
-  if (s.due) {
+  if (GATE) {
     const int64_t stamp = t0 + W - 1 + kDelay;
     for (int q = 0; q < np; ++q) {
       uint32_t sent = 0;
@@ -1448,6 +1523,7 @@ __global__ __launch_bounds__(kBlock) void k_reset_trials(fis_sim s,
     const int b = trials[q];
     if (b < 0 || b >= s.batch) continue;
     const int idx = i * bs + b;
+    unsettle(s, idx);
     s.v[idx] = v0;
     s.g[idx] = 0.0f;
     s.st[idx] = (uint8_t)min(s.refrac_steps, (int)FIS_ST_RMAX);
@@ -1475,6 +1551,7 @@ __global__ __launch_bounds__(kBlock) void k_flag_pairs(fis_sim s,
   const int x = idx[e];
   if (x < 0 || x >= s.n * s.bstride) return;
   const int sh = (x & 3) * 8;
+  unsettle(s, x);
   atomicOr((uint32_t*)(s.fl + (x & ~3)), (bits & 0xFFu) << sh);
   if (bits & FIS_NOREFRAC)
     atomicAnd((uint32_t*)(s.st + (x & ~3)), ~(0xFFu << sh));
@@ -1651,6 +1728,7 @@ int64_t fis_struct_layout(int what) {
     case 10: return (int64_t)offsetof(fis_sim, clock_ahead);
     case 11: return (int64_t)offsetof(fis_sim, drive_dense);
     case 12: return (int64_t)offsetof(fis_sim, due);
+    case 13: return (int64_t)offsetof(fis_sim, calm);
     default: return -1;
   }
 }
@@ -1810,11 +1888,13 @@ int fis_pull_window(const fis_sim* s, int W, int ring_steps, void* stream) {
   if (W < 1 || W > FIS_MAX_WINDOW || ring_steps < 0)
     return (int)hipErrorInvalidValue;
   const int tile = pull_tile_of(*s);
-  auto kern = k_pull<FIS_MAX_WINDOW, 256, 256>;
+  const bool gate = s->due != nullptr;
+  auto kern = k_pull<FIS_MAX_WINDOW, 256, 256, false>;
   switch (s->bstride * 4 + tile / 128) {  // tile / 128 is 1, 2 or 4
-#define FIS_PULL_CASE(B, T)                  \
-  case B * 4 + T / 128:                      \
-    kern = k_pull<FIS_MAX_WINDOW, B, T>;     \
+#define FIS_PULL_CASE(B, T)                                    \
+  case B * 4 + T / 128:                                        \
+    kern = gate ? k_pull<FIS_MAX_WINDOW, B, T, true>           \
+                : k_pull<FIS_MAX_WINDOW, B, T, false>;         \
     break;
     FIS_PULL_CASE(64, 128) FIS_PULL_CASE(64, 256) FIS_PULL_CASE(64, 512)
     FIS_PULL_CASE(128, 128) FIS_PULL_CASE(128, 256) FIS_PULL_CASE(128, 512)

@@ -235,11 +235,17 @@ class DeviceSim:
                                    dtype=torch.int64, **z)
             self.act = torch.zeros(n * max(1, bs // self.pull_tile),
                                    dtype=torch.int64, **z)
-            self.due = (torch.zeros(n * max(1, bs // self.pull_tile),
-                                    dtype=torch.int64, **z) if gate else None)
+            parts = max(1, bs // self.pull_tile)
+            self.due = (torch.zeros(n * parts, dtype=torch.int64, **z)
+                        if gate else None)
+            # calm (neuron, part)s skip windows unread; any host write of
+            # state or flags zeroes it (_calm_reset)
+            self.calm = (torch.zeros((n * parts + 3) // 4 * 4,
+                                     dtype=torch.uint8, **z)[:n * parts]
+                         if gate else None)
         else:
             self.order = self.in_offsets = self.in_cols = self.in_vals = None
-            self.rec = self.act = self.due = None
+            self.rec = self.act = self.due = self.calm = None
         self.gate = self.due is not None
         self.t = 0  # host mirror of the device clock
         self._open_window = 0
@@ -286,7 +292,7 @@ class DeviceSim:
         s.raster_len = self.raster_len
         for name in ("in_offsets", "in_cols", "in_vals", "rec", "act",
                      "order", "word_end", "drive_group", "drive_rate",
-                     "due"):
+                     "due", "calm"):
             t = getattr(self, name, None)
             setattr(s, name, t.data_ptr() if t is not None else None)
         dims = getattr(self, "_drive_dims", (0, 0, 0))
@@ -314,7 +320,14 @@ class DeviceSim:
             raise IndexError("neuron index outside [0, %d)" % self.n)
         return a
 
+    def _calm_reset(self):
+        """The host wrote state or flags: no (neuron, part) may skip its
+        next window unread (fis_sim.calm)."""
+        if getattr(self, "calm", None) is not None:
+            self.calm.zero_()
+
     def _push_flags(self):
+        self._calm_reset()
         self.fl.copy_(torch.from_numpy(self.fl_host.reshape(-1)))
 
     def set_flag(self, bit, neurons, trials=None):
@@ -375,7 +388,7 @@ class DeviceSim:
         tr = np.asarray(trials, dtype=np.int64).reshape(-1)
         if ne.size:
             self.fl_host[ne, tr] |= np.uint8(L.NOREFRAC)
-            self._push_flags()
+            self._push_flags()  # resets calm
             st = self.st.cpu().numpy().reshape(self.n, self.bstride)
             st[ne, tr] = 0
             self.st.copy_(torch.from_numpy(st.reshape(-1)))
@@ -416,6 +429,7 @@ class DeviceSim:
         tr = self._trials(trials)
         if tr.size == 0:
             return
+        self._calm_reset()
         sd = np.asarray(seeds, np.uint64).reshape(-1)
         te = np.asarray(t_end, np.int64).reshape(-1)
         if sd.size != tr.size or te.size != tr.size:
@@ -524,6 +538,7 @@ class DeviceSim:
                              % (state["n"], self.n))
         b = int(self._trials([trial])[0])
         n, bs = self.n, self.bstride
+        self._calm_reset()
         tt = torch.tensor([b], dtype=torch.int32, device=self.device)
         L.check(self.lib.fis_reset_trials(self._sp, _ptr(tt), 1,
                                           float(self.c["v0"]), _stream()),
@@ -571,6 +586,7 @@ class DeviceSim:
 
     def write_trials(self, buf, arr):
         """Write [batch, n] values into a trial-minor buffer as one copy."""
+        self._calm_reset()
         host = buf.cpu().numpy().reshape(self.n, self.bstride).copy()
         host[:, :self.batch] = np.asarray(arr).reshape(self.batch, self.n).T
         buf.copy_(torch.from_numpy(host.reshape(-1)))

@@ -499,7 +499,8 @@ class Engine:
         s = self._sim
         names = ["v", "g", "st", "ring", "sbits", "ev_count", "clock",
                  "spikes_out"]
-        names += [k for k in ("rec", "act") if getattr(s, k) is not None]
+        names += [k for k in ("rec", "act", "due", "calm")
+                  if getattr(s, k, None) is not None]
         return ({k: getattr(s, k).clone() for k in names}, s.t)
 
     def _device_restore(self, snap):
@@ -577,6 +578,7 @@ class Engine:
         present, else from ``calls``; it must agree with ``ptr``.
         """
         s = self._sim
+        s._calm_reset()  # state is written from the host below
         done = []
         B, n, bs = self.batch, self.n, s.bstride
         T = s.t

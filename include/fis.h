@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define FIS_ABI_VERSION 12
+#define FIS_ABI_VERSION 13
 #define FIS_MAX_WINDOW 20   /* steps per window: delay horizon 19 + 1     */
 #define FIS_RING_LEN 40     /* int32 ring slots, two delays: the spikes of
                                step u are summed into slot (u + 21) mod 40,
@@ -200,6 +200,17 @@ typedef struct {
    * synaptic input and skips the in-edge walk.  Zero at the start; fixed
    * for a simulation's life (on from its first window, or never). */
   int64_t* due;
+  /* Pull road, with `due` (nullable = off): [n][max(1, bstride /
+   * pull_tile)] bytes, nonzero where every neuron-trial of the (neuron,
+   * tile part) sat at a fixed point of the step through the last window
+   * (no input, no kick, no spike, g = +0, v unchanged by the leak step,
+   * refractory counter saturated, no flag).  Such a part with no input due
+   * and no kick in the next window skips it without reading its state:
+   * the window would change nothing but its record and act words.  Set
+   * and cleared by fis_pull_window; the drive entry points,
+   * fis_reset_trials and fis_flag_pairs clear the entries they touch.  A
+   * caller that writes v, g, st, fl or vhold itself zeroes it. */
+  uint8_t* calm;
 } fis_sim;
 
 /* Drive table entry: neuron-trial `idx` is driven at rate
@@ -355,8 +366,8 @@ int fis_flag_pairs(const fis_sim* s, const int32_t* idx, int n,
 /* sizeof / offsetof of the structs above (0: sizeof fis_sim, 1: offsets,
  * 2: w_scale, 3: refrac_steps, 4: raster_len, 5: sizeof fis_event,
  * 6: sizeof fis_drive_entry, 7: in_offsets, 8: t_origin, 9: pull_tile,
- * 10: clock_ahead, 11: drive_dense, 12: due), so a binding can check its
- * mirror. */
+ * 10: clock_ahead, 11: drive_dense, 12: due, 13: calm), so a binding can
+ * check its mirror. */
 int64_t fis_struct_layout(int what);
 
 /* Number of update waves (event segments) for a given n and bstride. */

@@ -494,3 +494,58 @@ def test_dense_rates_drawn_in_the_update_equal_the_drive_launch(batch):
     torch.testing.assert_close(fused.v, split.v, rtol=0, atol=0)
     torch.testing.assert_close(fused.g, split.g, rtol=0, atol=0)
     assert fused.read_raster(t - 20, t).any()
+
+
+@pytest.mark.parametrize("batch,tile", [(64, 256), (256, 256), (128, 128)])
+def test_calm_tiles_wake_up_exactly(batch, tile):
+    """Gated pull road: (neuron, part)s that settle at a fixed point go
+    calm and skip windows unread; then a drive switches on at step 300
+    (kicks reach calm neurons: the drive launch ends their calm) and
+    its spikes reach calm targets (input due ends the skip).  Raster and
+    state equal the oracle throughout; calm entries were set and later
+    cleared."""
+    conn = C.synthetic(640, None, mean_out=8, count_exp=1.2, n_groups=4,
+                       seed=31)
+    n = conn.n
+    rng = np.random.default_rng(batch + tile)
+    seeds = rng.integers(0, 2**63, size=batch, dtype=np.uint64)
+    # an early drive in four trials only: the parts of neurons it never
+    # reaches stay at rest in every trial and go calm
+    early = [rng.choice(n, 2 if b < 4 else 0, replace=False)
+             for b in range(batch)]
+    late = [rng.choice(n, 6, replace=False) for _ in range(batch)]
+    drv = [np.unique(np.concatenate([a, b])) for a, b in zip(early, late)]
+    net = DeviceNet(conn)
+    sim = DeviceSim(net, batch, seeds=seeds, raster_len=40, road="pull",
+                    pull_tile=tile, gate=True)
+    ne = np.concatenate(drv)
+    tr = np.concatenate([np.full(len(d), b) for b, d in enumerate(drv)])
+    sim.exempt_pairs(ne, tr)
+    # group 0: the early neurons at 60 Hz throughout; group 1: the late
+    # ones, off for 300 steps, then 700 Hz
+    grp = np.concatenate([np.where(np.isin(d, a), 0, 1)
+                          for d, a in zip(drv, early)])
+    table = DriveTable(sim, ne, tr, grp,
+                       np.array([[60.0, 0.0], [60.0, 0.0], [60.0, 0.0],
+                                 [60.0, 700.0]], np.float32), seg_steps=100)
+    ora = O.OracleEngine(conn, batch, seeds=seeds, no_refractory_trials=drv)
+    calm_seen = 0
+    steps = 800
+    for a in range(0, steps, 20):
+        sim.window(20, table)
+        raster = sim.read_raster(a, a + 20)
+        rates = np.zeros((batch, n), np.float32)
+        for k in range(20):
+            t = a + k
+            for b, d in enumerate(drv):
+                on = np.isin(d, early[b])
+                rates[b, d[on]] = 60.0
+                rates[b, d[~on]] = 700.0 if t >= 300 else 0.0
+            ora.step_rates(rates)
+            assert np.array_equal(raster[k], ora.spikes.astype(bool)), \
+                "raster differs at step %d" % t
+        if a + 20 == 300:
+            calm_seen = int(sim.calm.sum().item())
+    compare_state(sim, ora, "end")
+    assert calm_seen > 0, "no (neuron, part) went calm before the drive"
+    assert int(sim.calm.sum().item()) < calm_seen
