@@ -1023,7 +1023,18 @@ __device__ __forceinline__ Pair deal_pair(int pi, int excl, uint32_t m,
 // (each wave its own in-edges) 1.014; 4 (all waves) 0.975; 2 0.898 --
 // pairs of waves balance the typical 100-odd in-edges and still give each
 // wave two rounds to overlap.
-constexpr int kPullShare = 2;
+#ifndef FIS_PULL_SHARE
+#define FIS_PULL_SHARE 2
+#endif
+constexpr int kPullShare = FIS_PULL_SHARE;
+// Rounds of 64 dealt pairs a wave keeps in flight at once (owner searches
+// and record loads of the rounds overlap).  Measured at 138,639 neurons
+// (ms per window, drive included): 1 round 0.874 (B = 256) and 0.162
+// (B = 64), 2 rounds 0.878 / 0.172, 3 0.894 / 0.191, 4 1.09 / 0.213.
+#ifndef FIS_PULL_ROUNDS
+#define FIS_PULL_ROUNDS 1
+#endif
+constexpr int kRounds = FIS_PULL_ROUNDS;
 
 // Neuron-trials per workgroup (fis_sim.pull_tile; 0 = FIS_PULL_TILE).  256
 // lanes (4 waves, 20 KB of accumulators) measured fastest at 256 trials:
@@ -1242,29 +1253,53 @@ void k_pull(fis_sim s, int W,
       }
       // rounds of 64 pairs, two at a time (their owner searches and record
       // loads overlap); shared batches hand the rounds to the waves in turn
-      for (int base = mem * kWave; base < total; base += 2 * kRoundGap) {
-        Pair pr[2];
+      for (int base = mem * kWave; base < total; base += kRounds * kRoundGap) {
+        Pair pr[kRounds];
 #pragma unroll
-        for (int r = 0; r < 2; ++r)
+        for (int r = 0; r < kRounds; ++r)
           pr[r] = deal_pair(base + r * kRoundGap + lane, excl, m, src, val, pq);
-        uint64_t xw[2][wpt];
+        uint64_t xw[kRounds][wpt];
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
+        for (int r = 0; r < kRounds; ++r) {
           const bool ok = base + r * kRoundGap + lane < total;
           int sl = rbase + pr[r].step;
           sl = sl >= kRec ? sl - kRec : sl;
           const uint64_t* rw =
               s.rec + (int64_t)pr[r].src * pstride + sl * words + w0;
 #pragma unroll
-          for (int q = 0; q < wpt; ++q)
-            xw[r][q] = (ok && ((live_words >> q) & 1u)) ? rw[q] : 0ull;
+          for (int q = 0; q < wpt; ++q) xw[r][q] = 0ull;
+          if (ok) {
+            // the tile's words of the step: 8 * wpt contiguous bytes, 16-byte
+            // aligned from two words up (one vector load per 16 bytes)
+            if constexpr (wpt >= 2) {
+#pragma unroll
+              for (int q = 0; q < wpt; q += 2) {
+                const ulonglong2 w2 = *(const ulonglong2*)(rw + q);
+                xw[r][q] = w2.x;
+                xw[r][q + 1] = w2.y;
+              }
+            } else {
+              xw[r][0] = rw[0];
+            }
+          }
+          if (live_words != (1u << wpt) - 1u) {  // idle 64-trial words
+#pragma unroll
+            for (int q = 0; q < wpt; ++q)
+              if (!((live_words >> q) & 1u)) xw[r][q] = 0ull;
+          }
         }
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
+        for (int r = 0; r < kRounds; ++r) {
           int32_t* at = acc + pr[r].pos * tpn * WMAX + pr[r].step;
+          // the synapse count arrives by a lane shuffle: wait for it here,
+          // once, so that the LDS adds below do not each wait for the
+          // previous one (a wait placed at its first use inside the loop
+          // would cover the loop's own adds too)
+          int32_t cv = pr[r].val;
+          asm volatile("s_waitcnt lgkmcnt(0)" : "+v"(cv));
 #pragma unroll
           for (int q = 0; q < wpt; ++q)
-            add_bits<WMAX>(at + q * kWave * WMAX, xw[r][q], pr[r].val);
+            add_bits<WMAX>(at + q * kWave * WMAX, xw[r][q], cv);
         }
       }
       e = en;
