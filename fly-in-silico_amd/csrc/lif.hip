@@ -986,17 +986,40 @@ struct Pair {
   int step;     // step of the horizon (0 .. W-1)
 };
 
+// Inclusive max-scan over the 64 lanes of a wave, x >= 0 (DPP row shifts
+// within rows of 16, then the row broadcasts of lanes 15 and 31).
+__device__ __forceinline__ int wave_max_scan(int x) {
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, true));
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, true));
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, true));
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, true));
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false));
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false));
+  return x;
+}
+
+// The owner of pair `base + lane` (base uniform): the last lane L with
+// excl(L) <= base + lane.  The owner of pair `base` is the last lane whose
+// run starts at or before it (a ballot); every lane with pairs whose run
+// starts inside the round pushes its number to the lane of its start
+// (forward permute: the starts of lanes with pairs are distinct), and a
+// max-scan carries each start to the lanes after it.  No shuffle chain.
+__device__ __forceinline__ int round_owner(int base, int lane, int excl,
+                                           uint32_t m) {
+  const int carry = 63 - __builtin_clzll(__ballot(excl <= base));
+  const int d = excl - base;
+  const bool sends = m != 0u && d > 0 && d < kWave;
+  int got = __builtin_amdgcn_ds_permute((sends ? d : 0) << 2,
+                                        sends ? lane + 1 : 0);
+  if (lane == 0) got = carry + 1;  // only lanes without a start sent here
+  return wave_max_scan(got) - 1;
+}
+
 __device__ __forceinline__ Pair deal_pair(int pi, int excl, uint32_t m,
                                           int src, int32_t val, int pos) {
-  int o = 0, eo = 0;
-#pragma unroll
-  for (int st = kWave / 2; st > 0; st >>= 1) {
-    const int ex = __shfl(excl, o + st);
-    if (ex <= pi) {
-      o += st;
-      eo = ex;
-    }
-  }
+  const int lane = __lane_id();
+  const int o = round_owner(pi - lane, lane, excl, m);
+  const int eo = __shfl(excl, o);
   Pair p;
   const uint32_t mo = (uint32_t)__shfl((int)m, o);
   p.src = __shfl(src, o);
