@@ -91,8 +91,10 @@ def stages_of(runner):
 
 
 def kernel_times(runner, windows):
-    """Average durations (ms) of each launch of a window over ``windows``
-    eager windows, bracketed by HIP events on the launch stream."""
+    """Median durations (ms) of each launch of a window over ``windows``
+    eager windows, bracketed by HIP events on the launch stream (the
+    median: a window delayed by the host now and then must not stand for
+    the launch)."""
     import torch
 
     stages = stages_of(runner)
@@ -113,8 +115,11 @@ def kernel_times(runner, windows):
             marks[i + 1].record(stream)
             ev[k].append((marks[i], marks[i + 1]))
     torch.cuda.synchronize()
-    return {k: sum(a.elapsed_time(b) for a, b in v) / len(v)
-            for k, v in ev.items()}
+    out = {}
+    for k, v in ev.items():
+        d = sorted(a.elapsed_time(b) for a, b in v)
+        out[k] = d[len(d) // 2]
+    return out
 
 
 def popcount64(a):
@@ -589,11 +594,12 @@ def main():
     dom = max((k for k in kt if k != "drive"), key=lambda k: kt[k])
     hs = r.sim
     kname = {"update": "k_update", "update_scatter": "k_update",
-             "pull": "k_pull<%d, %d, %d>" % (W, hs.bstride, hs.pull_tile),
+             "pull": "k_pull<%d, %d, %d, %s>" % (W, hs.bstride, hs.pull_tile,
+                                                 "true" if hs.gate else "false"),
              "scatter": "k_scatter_steps"}[dom]
     spts = head["spikes"] / (B * head["steps"])
     roof = {"bound": "hbm", "kernel": kname, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "avg_launch_ms": round(kt[dom], 5)}
+            "unit": "GB/s", "launch_ms": round(kt[dom], 5)}
     if not args.no_model and dom in ("pull", "update_scatter"):
         wb, parts, extra = window_model(r, conn)
         achieved = wb / (kt[dom] * 1e-3) / 1e9
