@@ -137,9 +137,11 @@ def pull_window_bytes(sim, conn, W):
                   window's kick bits (4)
       in-edges    per tile part: 4 B source + 2 B count + 8 B act word per
                   in-edge, 4 B offset per neuron
-      records     8 B per 64-trial word of the tile part, per active
-                  (in-edge, horizon step) pair -- a step at which the
-                  source sent in some trial of the part
+      records     per active (in-edge, horizon step) pair -- a step at
+                  which the source sent in some trial of the part -- the
+                  part's 16-byte trial list (parts of 128 and 256 trials),
+                  plus 8 B per 64-trial word of the part where the list
+                  sends the reader to the words; 8 B per word otherwise
       own records W steps x bstride / 64 words x 8 B per neuron written,
                   and its act word read-modify-written (16 B per part)
       counts      8 B per neuron-trial that spiked in the window
@@ -164,7 +166,20 @@ def pull_window_bytes(sim, conn, W):
             hmask |= 1 << (u % L.REC_SLOTS)
     act = sim.act.cpu().numpy().reshape(n, parts).view(np.uint64)
     pairs_src = popcount64(act & np.uint64(hmask)).sum(axis=1)
-    pairs = int((conn.degree().astype(np.int64) * pairs_src).sum())
+    deg = conn.degree().astype(np.int64)
+    pairs = int((deg * pairs_src).sum())
+    lists = wpt >= 2 and tpn <= 256 and sim.rec_ids is not None
+    over = 0  # pairs whose list sends the reader to the words
+    if lists:
+        cnt = sim.rec_ids.cpu().numpy().reshape(n, parts, L.REC_SLOTS, 16)
+        cnt = cnt[:, :, :, 0]
+        for u in range(t0 - 20, t0 + W - 20):
+            if u < 0:
+                continue
+            j = u % L.REC_SLOTS
+            valid = ((act >> np.uint64(j)) & np.uint64(1)).astype(bool)
+            over += int((deg * (valid & (cnt[:, :, j] == 0xFF)).sum(axis=1)
+                         ).sum())
     # run the window, then count the neuron-trials that sent in it (record
     # words of a step are valid where the part's act bit for it is set)
     sim.pull(W)
@@ -181,11 +196,13 @@ def pull_window_bytes(sim, conn, W):
     parts_b = {
         "state": 23 * n * bs,
         "in_edges": parts * (14 * E + 4 * n),
-        "records_read": 8 * wpt * pairs,
+        "records_read": (16 * pairs + 8 * wpt * over) if lists
+                        else 8 * wpt * pairs,
         "records_written": 8 * W * words * n + 16 * n * parts,
         "counts": 8 * spiking,
     }
     return sum(parts_b.values()), parts_b, {"active_pairs": pairs,
+                                            "pairs_read_as_words": over,
                                             "spiking_neuron_trials": spiking}
 
 

@@ -250,6 +250,16 @@ __device__ __forceinline__ bool any_lane(bool p) {
   return m != 0;
 }
 
+// x, held in a vector register the compiler cannot see through: a load
+// whose address depends on it is a vector load, waited for only where its
+// value is used (a uniform address would be a scalar load, which the next
+// wait for LDS or another scalar load also waits for)
+__device__ __forceinline__ int opaque_vgpr(int x) {
+  int y;
+  asm("v_mov_b32 %0, %1" : "=v"(y) : "v"(x));
+  return y;
+}
+
 // OR of `x` over aligned groups of `width` lanes (width a power of two <= 64);
 // every lane of a group receives the group's value.
 __device__ __forceinline__ uint64_t group_or(uint64_t x, int width) {
@@ -974,6 +984,35 @@ __device__ __forceinline__ void add_bits(int32_t* acc, uint64_t x,
   }
 }
 
+// Record lists: 16 bytes per (neuron, tile part, record slot); byte 0 the
+// number of listed trials (<= FIS_REC_LIST_MAX) or FIS_REC_LIST_WORDS, bytes
+// 1.. the trials' numbers within the part.
+constexpr uint32_t kListMax = FIS_REC_LIST_MAX;
+
+// acc[t * stride] += val for the first `cnt` trials t of list l (one LDS add
+// per listed trial; the wave runs as many rounds as its longest list)
+template <int stride>
+__device__ __forceinline__ void add_list(int32_t* acc, const uint4& l,
+                                         uint32_t cnt, int32_t val) {
+#pragma unroll
+  for (int i = 0; i < (int)kListMax; ++i) {
+    if (!any_lane((uint32_t)i < cnt)) break;
+    if ((uint32_t)i < cnt) {
+      const int byte = i + 1;
+      const uint32_t d = byte < 4 ? l.x : (byte < 8 ? l.y : (byte < 12 ? l.z : l.w));
+      atomicAdd(acc + ((d >> ((byte & 3) * 8)) & 0xFFu) * stride, val);
+    }
+  }
+}
+
+// prepend byte b to the 16-byte list l (bytes move up by one)
+__device__ __forceinline__ void list_push(uint4& l, uint32_t b) {
+  l.w = (l.w << 8) | (l.z >> 24);
+  l.z = (l.z << 8) | (l.y >> 24);
+  l.y = (l.y << 8) | (l.x >> 24);
+  l.x = (l.x << 8) | b;
+}
+
 // One active (in-edge, step) pair of a wave's 64 in-edges, dealt to the
 // lane that takes pair number pi: lane L of the 64 owns pairs
 // excl(L) .. excl(L) + popc(m(L)) - 1, one per set step bit of m(L).  All
@@ -1098,6 +1137,10 @@ void k_pull(fis_sim s, int W,
   constexpr int parts = BS / tpn;               // tiles per neuron
   constexpr int words = BS >> 6;                // 64-trial words per neuron
   constexpr int wpt = tpn >> 6;                 // of them in one tile
+  // parts of 128 and 256 trials read the record lists (trial numbers fit a
+  // byte; below 128 trials a list is no shorter than the one word)
+  constexpr bool LISTS = wpt >= 2 && tpn <= 256;
+  const bool lists = LISTS && s.rec_ids != nullptr;
   const int tid = (int)threadIdx.x;
   // XCD-aware tile order: workgroup b is dispatched to XCD b mod 8, and
   // XCD x takes the consecutive tiles [x * per_xcd, (x + 1) * per_xcd) of
@@ -1132,9 +1175,11 @@ void k_pull(fis_sim s, int W,
   // this tile alone writes the act word of each of its neurons (for its
   // part); read now, updated at the end with a plain store if it changes
   // (other tiles read only the bits of the delay horizon, which the update
-  // keeps)
-  uint64_t act_old = 0;
-  if (tid < np) act_old = s.act[(int64_t)neuron_at(p0 + tid) * parts + part];
+  // keeps).  Every lane loads one (a vector load, its own address), so that
+  // nothing waits for it before the end.
+  const uint64_t act_old =
+      s.act[(int64_t)neuron_at(p0 + opaque_vgpr(min(tid, np - 1))) * parts +
+            part];
   // calm: every neuron of the tile sat at a fixed point through the last
   // window, and no input, kick or carried-over ring input reaches it in
   // this one -- the window changes nothing but the record and act words,
@@ -1159,6 +1204,10 @@ void k_pull(fis_sim s, int W,
         const int slot =
             wbase + lane < kRec ? wbase + lane : wbase + lane - kRec;
         s.rec[(int64_t)pre * pstride + slot * words + (tb >> 6)] = 0ull;
+        if (lists && tid - pl * tpn < kWave)  // the part's list: empty
+          *(uint4*)(s.rec_ids +
+                    (((int64_t)pre * parts + part) * kRec + slot) * 16) =
+              make_uint4(0u, 0u, 0u, 0u);
       }
       if (s.raster) {
         const int nblk = (s.n * bs + kBlock - 1) / kBlock;
@@ -1178,72 +1227,85 @@ void k_pull(fis_sim s, int W,
   }
 
 
-#pragma unroll
-  for (int j = 0; j < WMAX; j += 4)
-    *(int4*)(acc + tid * WMAX + j) = make_int4(0, 0, 0, 0);
-  // the tile's in-edge offsets: uniform, held in scalar registers
-  int off[P + 1];
-#pragma unroll
-  for (int q = 0; q <= P; ++q)
-    off[q] = q <= np ? s.in_offsets[p0 + q] : 0x7fffffff;
-  __syncthreads();
-
-  // the update's neuron-trial: its state is read now, so that the loads
-  // overlap the gather
+  // The loads that depend on nothing else are issued first, and nothing
+  // waits for them before its result is needed: the update's state, the
+  // tile's in-edge offsets, the word ends.
   const int nb = s.n * bs;
   const int lane = tid & (kWave - 1);
   const int pl = tid / tpn;                          // position in the tile
   const int tb = (w0 << 6) + (tid - pl * tpn);       // trial of this lane
-  // uniform over a wave: a neuron of the tile, in a live 64-trial word
-  const bool act = pl < np && (!s.word_end || t0 < s.word_end[tb >> 6]);
   const int pre = pl < np ? neuron_at(p0 + pl) : 0;  // neuron of this lane
   const int idx = pre * bs + tb;
   const int wave = idx >> 6;
   const int nblk = (nb + kBlock - 1) / kBlock;  // raster rows: 256-lane groups
   float v = 0.0f, g = 0.0f, vh = 0.0f;
   uint32_t st = 0, fl = FIS_FIRE_OFF, sb = 0;
-  if (act) {
+  if (pl < np) {  // in bounds whether or not the trial's word is live
     v = s.v[idx];
     g = s.g[idx];
     st = s.st[idx];
     fl = s.fl[idx];
     sb = s.sbits[idx];
   }
-
-  if (gather) {  // 1. gather.  The tile's in-edges are read 64 at a time, one per lane,
-     // with their presynaptic act words (the next 64 are fetched
-     // meanwhile).  The active (in-edge, step) pairs of those 64 -- a step
-     // of the horizon at which the presynaptic neuron spiked in some trial
-     // of the tile -- are then dealt out 64 at a time, one per lane: the
-     // lane reads that step's record words of the tile's trials and adds
-     // the synapse count at every set trial bit.  A lane thus carries a few
-     // trial bits of one spike of one neuron, and the walk over them is as
-     // long as the densest such pair of the round (a handful of bits), not
-     // as long as the densest of 20 steps' words of the in-edge.
-    const int wv = tid >> 6;                 // wave of the tile
-    constexpr int NW = TILE / kWave;         // waves per tile
-    uint32_t live_words = 0;                 // tile words still simulated
+  // the tile's in-edge offsets: uniform, held in scalar registers
+  int off[P + 1];
+#pragma unroll
+  for (int q = 0; q <= P; ++q)
+    off[q] = q <= np ? s.in_offsets[p0 + q] : 0x7fffffff;
+  // uniform over a wave: a neuron of the tile, in a live 64-trial word
+  const bool act = pl < np && (!s.word_end || t0 < s.word_end[tb >> 6]);
+  uint32_t live_words = (1u << wpt) - 1u;  // tile words still simulated
+  if (s.word_end) {
+    int64_t we[wpt];
+#pragma unroll
+    for (int j = 0; j < wpt; ++j) we[j] = s.word_end[w0 + j];
+    live_words = 0;
 #pragma unroll
     for (int j = 0; j < wpt; ++j)
-      if (!s.word_end || t0 < s.word_end[w0 + j]) live_words |= 1u << j;
-    const int e1 = s.in_offsets[p0 + np];
-    // one neuron per tile: the waves form groups of SG that read the same
-    // 64 in-edges and share their rounds of pairs (balanced however few the
-    // edges); several neurons per tile: each wave takes its own 64 in-edges
-    constexpr int SG = P == 1 ? kPullShare : 1;  // waves per group
-    constexpr int NG = NW / SG;                  // groups per tile
-    const int grp = wv / SG, mem = wv % SG;
-    constexpr int kEdgeStep = NG * kWave;
-    constexpr int kRoundGap = SG * kWave;
-    int e = off[0] + grp * kWave + lane;  // this lane's in-edge
-    int src = 0;
-    int32_t val = 0;
-    uint64_t a = 0;  // act word of the presynaptic neuron
-    if (e < e1) {
-      src = s.in_cols[e];
-      val = s.in_vals[e];
-      a = s.act[src * parts + part];
-    }
+      if (t0 < we[j]) live_words |= 1u << j;
+  }
+
+#pragma unroll
+  for (int j = 0; j < WMAX; j += 4)
+    *(int4*)(acc + tid * WMAX + j) = make_int4(0, 0, 0, 0);
+
+  // 1. gather.  The tile's in-edges are read 64 at a time, one per lane,
+  // with their presynaptic act words (the next 64 are fetched meanwhile;
+  // the first 64 before the barrier that ends the LDS clear).  The active
+  // (in-edge, step) pairs of those 64 -- a step of the horizon at which the
+  // presynaptic neuron spiked in some trial of the tile -- are then dealt
+  // out 64 at a time, one per lane: the lane reads that step's record words
+  // of the tile's trials and adds the synapse count at every set trial bit.
+  // A lane thus carries a few trial bits of one spike of one neuron, and
+  // the walk over them is as long as the densest such pair of the round (a
+  // handful of bits), not as long as the densest of 20 steps' words of the
+  // in-edge.
+  const int wv = tid >> 6;                 // wave of the tile
+  constexpr int NW = TILE / kWave;         // waves per tile
+  int e1 = off[0];                         // end of the tile's in-edges
+#pragma unroll
+  for (int q = 1; q <= P; ++q)
+    if (q <= np) e1 = off[q];
+  // one neuron per tile: the waves form groups of SG that read the same
+  // 64 in-edges and share their rounds of pairs (balanced however few the
+  // edges); several neurons per tile: each wave takes its own 64 in-edges
+  constexpr int SG = P == 1 ? kPullShare : 1;  // waves per group
+  constexpr int NG = NW / SG;                  // groups per tile
+  const int grp = wv / SG, mem = wv % SG;
+  constexpr int kEdgeStep = NG * kWave;
+  constexpr int kRoundGap = SG * kWave;
+  int e = off[0] + grp * kWave + lane;  // this lane's in-edge
+  int src = 0;
+  int32_t val = 0;
+  uint64_t a = 0;  // act word of the presynaptic neuron
+  if (gather && e < e1) {
+    src = s.in_cols[e];
+    val = s.in_vals[e];
+    a = s.act[src * parts + part];
+  }
+  __syncthreads();
+
+  if (gather) {
     const int rot = (kRec - rbase) % kRec;
     // uniform over the wave: its first in-edge is below e1
     while (e - lane < e1) {
@@ -1281,48 +1343,94 @@ void k_pull(fis_sim s, int W,
 #pragma unroll
         for (int r = 0; r < kRounds; ++r)
           pr[r] = deal_pair(base + r * kRoundGap + lane, excl, m, src, val, pq);
-        uint64_t xw[kRounds][wpt];
+        if constexpr (LISTS) {
+          // the step's list of the part: 16 bytes per pair; a lane whose
+          // list overflowed (or was marked) reads the record words
+          uint4 li[kRounds];
 #pragma unroll
-        for (int r = 0; r < kRounds; ++r) {
-          const bool ok = base + r * kRoundGap + lane < total;
-          int sl = rbase + pr[r].step;
-          sl = sl >= kRec ? sl - kRec : sl;
-          const uint64_t* rw =
-              s.rec + (int64_t)pr[r].src * pstride + sl * words + w0;
+          for (int r = 0; r < kRounds; ++r) {
+            const bool ok = base + r * kRoundGap + lane < total;
+            int sl = rbase + pr[r].step;
+            sl = sl >= kRec ? sl - kRec : sl;
+            li[r] = make_uint4(ok ? FIS_REC_LIST_WORDS : 0u, 0u, 0u, 0u);
+            if (ok && lists)
+              li[r] = *(const uint4*)(s.rec_ids +
+                                      (((int64_t)pr[r].src * parts + part) *
+                                           kRec + sl) * 16);
+          }
 #pragma unroll
-          for (int q = 0; q < wpt; ++q) xw[r][q] = 0ull;
-          if (ok) {
-            // the tile's words of the step: 8 * wpt contiguous bytes, 16-byte
-            // aligned from two words up (one vector load per 16 bytes)
-            if constexpr (wpt >= 2) {
+          for (int r = 0; r < kRounds; ++r) {
+            int32_t* at = acc + pr[r].pos * tpn * WMAX + pr[r].step;
+            int32_t cv = pr[r].val;
+            asm volatile("s_waitcnt lgkmcnt(0)" : "+v"(cv));
+            const uint32_t cnt = li[r].x & 0xFFu;
+            const bool fb = cnt > kListMax;
+            if (any_lane(fb)) {  // the record words of those lanes
+              uint64_t xw[wpt];
 #pragma unroll
-              for (int q = 0; q < wpt; q += 2) {
-                const ulonglong2 w2 = *(const ulonglong2*)(rw + q);
-                xw[r][q] = w2.x;
-                xw[r][q + 1] = w2.y;
+              for (int q = 0; q < wpt; ++q) xw[q] = 0ull;
+              if (fb) {
+                int sl = rbase + pr[r].step;
+                sl = sl >= kRec ? sl - kRec : sl;
+                const uint64_t* rw =
+                    s.rec + (int64_t)pr[r].src * pstride + sl * words + w0;
+#pragma unroll
+                for (int q = 0; q < wpt; q += 2) {
+                  const ulonglong2 w2 = *(const ulonglong2*)(rw + q);
+                  xw[q] = (live_words >> q) & 1u ? w2.x : 0ull;
+                  xw[q + 1] = (live_words >> (q + 1)) & 1u ? w2.y : 0ull;
+                }
               }
-            } else {
-              xw[r][0] = rw[0];
+#pragma unroll
+              for (int q = 0; q < wpt; ++q)
+                add_bits<WMAX>(at + q * kWave * WMAX, xw[q], cv);
+            }
+            add_list<WMAX>(at, li[r], fb ? 0u : cnt, cv);
+          }
+        } else {
+          uint64_t xw[kRounds][wpt];
+#pragma unroll
+          for (int r = 0; r < kRounds; ++r) {
+            const bool ok = base + r * kRoundGap + lane < total;
+            int sl = rbase + pr[r].step;
+            sl = sl >= kRec ? sl - kRec : sl;
+            const uint64_t* rw =
+                s.rec + (int64_t)pr[r].src * pstride + sl * words + w0;
+#pragma unroll
+            for (int q = 0; q < wpt; ++q) xw[r][q] = 0ull;
+            if (ok) {
+              // the tile's words of the step: 8 * wpt contiguous bytes, 16-byte
+              // aligned from two words up (one vector load per 16 bytes)
+              if constexpr (wpt >= 2) {
+#pragma unroll
+                for (int q = 0; q < wpt; q += 2) {
+                  const ulonglong2 w2 = *(const ulonglong2*)(rw + q);
+                  xw[r][q] = w2.x;
+                  xw[r][q + 1] = w2.y;
+                }
+              } else {
+                xw[r][0] = rw[0];
+              }
+            }
+            if (live_words != (1u << wpt) - 1u) {  // idle 64-trial words
+#pragma unroll
+              for (int q = 0; q < wpt; ++q)
+                if (!((live_words >> q) & 1u)) xw[r][q] = 0ull;
             }
           }
-          if (live_words != (1u << wpt) - 1u) {  // idle 64-trial words
+#pragma unroll
+          for (int r = 0; r < kRounds; ++r) {
+            int32_t* at = acc + pr[r].pos * tpn * WMAX + pr[r].step;
+            // the synapse count arrives by a lane shuffle: wait for it here,
+            // once, so that the LDS adds below do not each wait for the
+            // previous one (a wait placed at its first use inside the loop
+            // would cover the loop's own adds too)
+            int32_t cv = pr[r].val;
+            asm volatile("s_waitcnt lgkmcnt(0)" : "+v"(cv));
 #pragma unroll
             for (int q = 0; q < wpt; ++q)
-              if (!((live_words >> q) & 1u)) xw[r][q] = 0ull;
+              add_bits<WMAX>(at + q * kWave * WMAX, xw[r][q], cv);
           }
-        }
-#pragma unroll
-        for (int r = 0; r < kRounds; ++r) {
-          int32_t* at = acc + pr[r].pos * tpn * WMAX + pr[r].step;
-          // the synapse count arrives by a lane shuffle: wait for it here,
-          // once, so that the LDS adds below do not each wait for the
-          // previous one (a wait placed at its first use inside the loop
-          // would cover the loop's own adds too)
-          int32_t cv = pr[r].val;
-          asm volatile("s_waitcnt lgkmcnt(0)" : "+v"(cv));
-#pragma unroll
-          for (int q = 0; q < wpt; ++q)
-            add_bits<WMAX>(at + q * kWave * WMAX, xw[r][q], cv);
         }
       }
       e = en;
@@ -1334,6 +1442,13 @@ void k_pull(fis_sim s, int W,
   __syncthreads();
 
   // 2. update
+  if (!act) {  // a lane past the tile's neurons or in a word that has ended
+    v = 0.0f;
+    g = 0.0f;
+    st = 0;
+    fl = FIS_FIRE_OFF;
+    sb = 0;
+  }
   if (sb) s.sbits[idx] = 0u;  // the window's kicks are consumed
   // the clamp value is read only where the flag says HOLD (by now fl has
   // long arrived; 4 B per neuron-trial and window not read otherwise)
@@ -1446,6 +1561,32 @@ void k_pull(fis_sim s, int W,
                (wave & 3)] = cell[1];
     }
   }
+  // The record lists, stored with the words: lane k of the first wave of
+  // each neuron of the tile lists the trials of its part that sent at step
+  // k (the words of the part's waves, all in LDS by now).  Waves of ended
+  // words sent nothing, so a list holds live trials only.
+  if (lists && tid - pl * tpn < kWave && pl < np && lane < W &&
+      (((part_sent >> lane) & 1u) || lane == W - 1)) {
+    uint4 l = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int q = 0; q < wpt; ++q) {
+      uint64_t x =
+          *(const uint64_t*)(acc + (pl * tpn + q * kWave) * WMAX + 4 * lane);
+      while (x) {
+        list_push(l, (uint32_t)(q * kWave + __ffsll((long long)x) - 1));
+        x &= x - 1ull;
+        ++cnt;
+      }
+    }
+    if (cnt <= kListMax)
+      list_push(l, cnt);
+    else
+      l = make_uint4(FIS_REC_LIST_WORDS, 0u, 0u, 0u);
+    const int slot = wbase + lane < kRec ? wbase + lane : wbase + lane - kRec;
+    *(uint4*)(s.rec_ids + (((int64_t)pre * parts + part) * kRec + slot) * 16) =
+        l;
+  }
 
   // 3. act bits of the window's steps: clear, then set
   if (tid < np) {
@@ -1509,13 +1650,18 @@ __global__ __launch_bounds__(kBlock) void k_pull_resend(fis_sim s,
   const int slot = (int)((now(s) + kRec - 1) % kRec);
   uint64_t* w = s.rec + ((int64_t)pre * kRec + slot) * words + (b >> 6);
   const uint64_t bit = 1ull << (b & 63);
+  const int tile = pull_tile_of(s);
+  const int parts = pull_parts(s.bstride, tile);
+  const int part = b / pull_trials_per_part(s.bstride, tile);
+  // the part's list of that step no longer matches its words: readers are
+  // sent to the words (every writer stores the same byte)
+  if (s.rec_ids)
+    s.rec_ids[(((int64_t)pre * parts + part) * kRec + slot) * 16] =
+        (uint8_t)FIS_REC_LIST_WORDS;
   if (s.fl[src] & FIS_OUT_OFF) {
     __hip_atomic_fetch_and(w, ~bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   } else {
     __hip_atomic_fetch_or(w, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int tile = pull_tile_of(s);
-    const int parts = pull_parts(s.bstride, tile);
-    const int part = b / pull_trials_per_part(s.bstride, tile);
     __hip_atomic_fetch_or(s.act + (int64_t)pre * parts + part, 1ull << slot,
                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (s.due) {  // the spike of step now - 1 now reaches its targets
@@ -1595,6 +1741,13 @@ __global__ __launch_bounds__(kBlock) void k_reset_trials(fis_sim s,
       uint64_t* r = s.rec + (int64_t)i * kRec * words + (b >> 6);
       const uint64_t keep = ~(1ull << (b & 63));
       for (int k = 0; k < kRec; ++k) r[k * words] &= keep;
+      if (s.rec_ids) {  // the lists of the trial's part: read the words
+        const int tile = pull_tile_of(s);
+        const int parts = pull_parts(bs, tile);
+        const int part = b / pull_trials_per_part(bs, tile);
+        uint8_t* li = s.rec_ids + ((int64_t)i * parts + part) * kRec * 16;
+        for (int k = 0; k < kRec; ++k) li[k * 16] = (uint8_t)FIS_REC_LIST_WORDS;
+      }
     }
   }
 }
@@ -1787,6 +1940,7 @@ int64_t fis_struct_layout(int what) {
     case 11: return (int64_t)offsetof(fis_sim, drive_dense);
     case 12: return (int64_t)offsetof(fis_sim, due);
     case 13: return (int64_t)offsetof(fis_sim, calm);
+    case 14: return (int64_t)offsetof(fis_sim, rec_ids);
     default: return -1;
   }
 }

@@ -20,7 +20,7 @@ CSRC = PKG.parent / "csrc"
 SOURCES = [CSRC / "lif.hip"]
 HEADER = ROOT / "include" / "fis.h"
 LIB_PATH = PKG / "libfis.so"
-ABI_VERSION = 13
+ABI_VERSION = 14
 
 MAX_WINDOW = 20
 SCRATCH_BYTES = 64 << 20  # FIS_SCRATCH_BYTES: fis_window_noalloc's staging
@@ -105,7 +105,7 @@ class FisSim(ctypes.Structure):
         ("drive_seg_steps", ctypes.c_int32),
         ("pull_tile", ctypes.c_int32),
         ("clock_ahead", ctypes.c_int32), ("defer_tick", ctypes.c_int32),
-        ("drive_dense", P), ("due", P), ("calm", P),
+        ("drive_dense", P), ("due", P), ("calm", P), ("rec_ids", P),
     ]
 
 
@@ -174,7 +174,8 @@ def lib():
             6: ctypes.sizeof(FisDriveEntry), 7: FisSim.in_offsets.offset,
             8: FisSim.t_origin.offset, 9: FisSim.pull_tile.offset,
             10: FisSim.clock_ahead.offset, 11: FisSim.drive_dense.offset,
-            12: FisSim.due.offset, 13: FisSim.calm.offset}
+            12: FisSim.due.offset, 13: FisSim.calm.offset,
+            14: FisSim.rec_ids.offset}
     for k, v in want.items():
         got = L.fis_struct_layout(k)
         if got != v:

@@ -86,7 +86,7 @@ class BatchRunner:
         s = self.sim
         names = ["v", "g", "st", "ring", "sbits", "ev_count", "clock"]
         names += [k for k in ("counts", "pop_counts", "raster", "rec", "act",
-                              "due", "calm")
+                              "due", "calm", "rec_ids")
                   if getattr(s, k) is not None]
         saved = {k: getattr(s, k).clone() for k in names}
         t0 = s.t

@@ -235,6 +235,11 @@ class DeviceSim:
                                    dtype=torch.int64, **z)
             self.act = torch.zeros(n * max(1, bs // self.pull_tile),
                                    dtype=torch.int64, **z)
+            # the records again as short trial lists, 16 B per (neuron,
+            # tile part, slot); zero = empty lists, matching zero words
+            self.rec_ids = torch.zeros(n * max(1, bs // self.pull_tile)
+                                       * L.REC_SLOTS * 16,
+                                       dtype=torch.uint8, **z)
             parts = max(1, bs // self.pull_tile)
             self.due = (torch.zeros(n * parts, dtype=torch.int64, **z)
                         if gate else None)
@@ -246,6 +251,7 @@ class DeviceSim:
         else:
             self.order = self.in_offsets = self.in_cols = self.in_vals = None
             self.rec = self.act = self.due = self.calm = None
+            self.rec_ids = None
         self.gate = self.due is not None
         self.t = 0  # host mirror of the device clock
         self._open_window = 0
@@ -292,7 +298,7 @@ class DeviceSim:
         s.raster_len = self.raster_len
         for name in ("in_offsets", "in_cols", "in_vals", "rec", "act",
                      "order", "word_end", "drive_group", "drive_rate",
-                     "due", "calm"):
+                     "due", "calm", "rec_ids"):
             t = getattr(self, name, None)
             setattr(s, name, t.data_ptr() if t is not None else None)
         dims = getattr(self, "_drive_dims", (0, 0, 0))

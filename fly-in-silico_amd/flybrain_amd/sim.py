@@ -499,7 +499,7 @@ class Engine:
         s = self._sim
         names = ["v", "g", "st", "ring", "sbits", "ev_count", "clock",
                  "spikes_out"]
-        names += [k for k in ("rec", "act", "due", "calm")
+        names += [k for k in ("rec", "act", "due", "calm", "rec_ids")
                   if getattr(s, k, None) is not None]
         return ({k: getattr(s, k).clone() for k in names}, s.t)
 
@@ -671,6 +671,7 @@ class Engine:
             # everything in flight is now in the int32 ring, which the next
             # 20 steps consume; the spike records hold nothing older
             s.rec.zero_()
+            s.rec_ids.zero_()
             s.act.zero_()
             s.ring_until = T + L.DELAY
 

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define FIS_ABI_VERSION 13
+#define FIS_ABI_VERSION 14
 #define FIS_MAX_WINDOW 20   /* steps per window: delay horizon 19 + 1     */
 #define FIS_RING_LEN 40     /* int32 ring slots, two delays: the spikes of
                                step u are summed into slot (u + 21) mod 40,
@@ -53,6 +53,8 @@ extern "C" {
 #define FIS_LINE_INTS 32    /* 128 bytes = 32 x 4-byte ints               */
 #define FIS_EVENT_ALIGN 8   /* 8 events = 128 bytes                       */
 #define FIS_REC_SLOTS 40    /* pull road: spike records of steps t mod 40 */
+#define FIS_REC_LIST_MAX 15     /* trials a record list holds */
+#define FIS_REC_LIST_WORDS 0xFF /* list count byte: read the record words */
 #define FIS_PULL_TILE 256   /* pull road: neuron-trials per workgroup:
                                256 / bstride neurons, or a quarter (512:
                                half) of one neuron's trials               */
@@ -211,6 +213,19 @@ typedef struct {
    * fis_reset_trials and fis_flag_pairs clear the entries they touch.  A
    * caller that writes v, g, st, fl or vhold itself zeroes it. */
   uint8_t* calm;
+  /* Pull road, nullable: the spike records again as short lists,
+   * [n][max(1, bstride / pull_tile)][FIS_REC_SLOTS][16] bytes, one per
+   * (neuron, tile part, record slot): byte 0 the number of trials of the
+   * part that sent at that step (0..15), or FIS_REC_LIST_WORDS (0xFF):
+   * read the record words instead; bytes 1.. their trial numbers within
+   * the part, in any order.  Valid exactly where the record words are.
+   * Written by fis_pull_window with the words; fis_pull_resend and
+   * fis_reset_trials mark what they change 0xFF.  Read in place of the
+   * words by parts of 128 and 256 trials (16 bytes per active pair
+   * instead of 8 per 64 trials, and one LDS add per listed trial).  Zero
+   * (empty lists, matching zero words) at the start; a caller that writes
+   * rec itself zeroes the lists with it or fills them with 0xFF. */
+  uint8_t* rec_ids;
 } fis_sim;
 
 /* Drive table entry: neuron-trial `idx` is driven at rate
@@ -366,8 +381,8 @@ int fis_flag_pairs(const fis_sim* s, const int32_t* idx, int n,
 /* sizeof / offsetof of the structs above (0: sizeof fis_sim, 1: offsets,
  * 2: w_scale, 3: refrac_steps, 4: raster_len, 5: sizeof fis_event,
  * 6: sizeof fis_drive_entry, 7: in_offsets, 8: t_origin, 9: pull_tile,
- * 10: clock_ahead, 11: drive_dense, 12: due, 13: calm), so a binding can
- * check its mirror. */
+ * 10: clock_ahead, 11: drive_dense, 12: due, 13: calm, 14: rec_ids), so a
+ * binding can check its mirror. */
 int64_t fis_struct_layout(int what);
 
 /* Number of update waves (event segments) for a given n and bstride. */

@@ -193,7 +193,10 @@ def test_pull_gated_matches_oracle(batch, tile, n_drive):
 
 
 def philox_batched(batch, road, fused, tile=None, params=None, gate=False,
-                   n_drive=(5, 40), conn_kw=None):
+                   n_drive=(5, 40), conn_kw=None, alike=False, lists=True):
+    """``alike``: every trial the same seed and drive (each spike is then
+    one of every trial: the record lists overflow); ``lists=False``: the
+    pull road reads the record words only."""
     kw = dict(mean_out=40, count_exp=1.2, n_groups=4, seed=21)
     kw.update(conn_kw or {})
     conn = C.synthetic(512, None, **kw)
@@ -203,9 +206,16 @@ def philox_batched(batch, road, fused, tile=None, params=None, gate=False,
     drv = [rng.choice(n, rng.integers(*n_drive), replace=False)
            for _ in range(batch)]
     rate = [float(rng.uniform(50, 900)) for _ in range(batch)]
+    if alike:
+        seeds[:] = seeds[0]
+        drv = [drv[0]] * batch
+        rate = [rate[0]] * batch
     net = DeviceNet(conn)
     sim = DeviceSim(net, batch, params=params, seeds=seeds, raster_len=40,
                     road=road, pull_tile=tile, gate=gate)
+    if not lists:
+        sim.rec_ids = None
+        sim._sync_struct()
     if params:
         assert sim.c["refrac_steps"] == round(params["tRefrac"] / 0.1)
     ne = np.concatenate(drv)
@@ -440,7 +450,7 @@ def test_outputs_stay_inside_their_buffers(road, batch):
     names = ["events", "ev_count", "raster", "v", "g", "sbits", "counts",
              "ring"]
     if road == "pull":
-        names += ["rec", "act"]
+        names += ["rec", "act", "rec_ids"]
     for name in names:
         t = getattr(sim, name)
         nbytes = t.numel() * t.element_size()
@@ -549,3 +559,59 @@ def test_calm_tiles_wake_up_exactly(batch, tile):
     compare_state(sim, ora, "end")
     assert calm_seen > 0, "no (neuron, part) went calm before the drive"
     assert int(sim.calm.sum().item()) < calm_seen
+
+
+def check_record_lists(sim):
+    """Every valid record list (act bit set, count <= 15) names exactly the
+    trials whose record-word bits are set; returns (lists checked, lists
+    that send readers to the words)."""
+    n, bs = sim.n, sim.bstride
+    tpn = min(bs, sim.pull_tile)
+    parts, wpt, words = bs // tpn, tpn // 64, bs // 64
+    rec = sim.rec.cpu().numpy().view(np.uint64).reshape(n, 40, words)
+    act = sim.act.cpu().numpy().view(np.uint64).reshape(n, parts)
+    ids = sim.rec_ids.cpu().numpy().reshape(n, parts, 40, 16)
+    checked = to_words = 0
+    for slot in range(40):
+        valid = ((act >> np.uint64(slot)) & np.uint64(1)).astype(bool)
+        for part in range(parts):
+            for i in np.nonzero(valid[:, part])[0]:
+                w = rec[i, slot, part * wpt:(part + 1) * wpt]
+                want = sorted(q * 64 + b for q in range(wpt) for b in range(64)
+                              if (int(w[q]) >> b) & 1)
+                cnt = int(ids[i, part, slot, 0])
+                if cnt == 0xFF:
+                    to_words += 1
+                    continue
+                got = sorted(ids[i, part, slot, 1:1 + cnt].tolist())
+                assert got == want, (i, part, slot, got, want)
+                checked += 1
+    return checked, to_words
+
+
+@pytest.mark.parametrize("batch,tile", [(128, 128), (256, 256), (256, 128),
+                                        (300, 256), (600, 256)])
+def test_record_lists_name_the_record_words(batch, tile):
+    """Parts of 128 and 256 trials read short trial lists in place of the
+    record words: after 400 steps against the oracle, every valid list
+    names exactly the trials its words hold."""
+    sim = philox_batched(batch, "pull", False, tile)
+    checked, _ = check_record_lists(sim)
+    assert checked > 0
+
+
+@pytest.mark.parametrize("batch,tile", [(256, 256), (128, 128), (600, 256)])
+def test_record_lists_overflow_to_the_words(batch, tile):
+    """Every trial alike: each spike is one of all the part's trials, no
+    list holds them and every reader takes the words; raster and state
+    still equal the oracle."""
+    sim = philox_batched(batch, "pull", False, tile, alike=True)
+    checked, to_words = check_record_lists(sim)
+    assert to_words > 0
+
+
+@pytest.mark.parametrize("batch,tile", [(256, 256), (300, 256)])
+def test_pull_road_without_record_lists(batch, tile):
+    """rec_ids is optional: without it the pull road reads the record
+    words, with the same raster and state."""
+    philox_batched(batch, "pull", False, tile, lists=False)
