@@ -1220,9 +1220,8 @@ void k_pull(fis_sim s, int W, int ring_steps, int n_tiles, int per_xcd) {
     int32_t val;
     uint64_t a;           // and its act word
   };
-  // the loads that depend on nothing else are issued first, and nothing
-  // waits for them before its result is needed
-  auto begin = [&]() {
+  // the tile, whether it is calm, and its act words
+  auto head = [&]() {
     Tile T;
     T.p0 = (tile / parts) * P;   // first position of the tile
     T.part = tile % parts;       // which part of its neuron(s)
@@ -1263,6 +1262,15 @@ void k_pull(fis_sim s, int W, int ring_steps, int n_tiles, int per_xcd) {
       T.calm = !T.gather && ring_steps == 0 &&
                T.calm_old == (1u << T.np) - 1u;
     }
+    T.tb = (T.w0 << 6) + tl;
+    return T;
+  };
+  // the rest of what the window reads before its gather: the loads that
+  // depend on nothing else are issued first, and nothing waits for them
+  // before its result is needed
+  auto load = [&](Tile& T) {
+    T.pre = pl < T.np ? neuron_at(T.p0 + pl) : 0;
+    T.idx = T.pre * bs + T.tb;
     T.live = (1u << wpt) - 1u;
     if (s.word_end) {
       int64_t we[wpt];
@@ -1273,9 +1281,6 @@ void k_pull(fis_sim s, int W, int ring_steps, int n_tiles, int per_xcd) {
       for (int j = 0; j < wpt; ++j)
         if (t0 < we[j]) T.live |= 1u << j;
     }
-    T.tb = (T.w0 << 6) + tl;
-    T.pre = pl < T.np ? neuron_at(T.p0 + pl) : 0;
-    T.idx = T.pre * bs + T.tb;
     T.v = 0.0f;
     T.g = 0.0f;
     T.st = 0;
@@ -1286,38 +1291,66 @@ void k_pull(fis_sim s, int W, int ring_steps, int n_tiles, int per_xcd) {
     T.a = 0;
 #pragma unroll
     for (int q = 0; q <= P; ++q) T.off[q] = 0x7fffffff;
-    T.e1 = 0;
-    if (!T.calm) {
-      if (pl < T.np) {  // in bounds whether or not the trial's word is live
-        T.v = s.v[T.idx];
-        T.g = s.g[T.idx];
-        T.st = s.st[T.idx];
-        T.fl = s.fl[T.idx];
-        T.sb = s.sbits[T.idx];
-      }
-      // the tile's in-edge offsets: uniform, held in scalar registers
-#pragma unroll
-      for (int q = 0; q <= P; ++q)
-        T.off[q] = q <= T.np ? s.in_offsets[T.p0 + q] : 0x7fffffff;
-      T.e1 = T.off[0];
-#pragma unroll
-      for (int q = 1; q <= P; ++q)
-        if (q <= T.np) T.e1 = T.off[q];
+    if (pl < T.np) {  // in bounds whether or not the trial's word is live
+      T.v = s.v[T.idx];
+      T.g = s.g[T.idx];
+      T.st = s.st[T.idx];
+      T.fl = s.fl[T.idx];
+      T.sb = s.sbits[T.idx];
     }
-    return T;
+    // the tile's in-edge offsets: uniform, held in scalar registers
+#pragma unroll
+    for (int q = 0; q <= P; ++q)
+      T.off[q] = q <= T.np ? s.in_offsets[T.p0 + q] : 0x7fffffff;
+    T.e1 = T.off[0];
+#pragma unroll
+    for (int q = 1; q <= P; ++q)
+      if (q <= T.np) T.e1 = T.off[q];
   };
   // the lane's first in-edge, then its act word
   auto first_edge = [&](Tile& T) {
     const int e = T.off[0] + grp * kWave + lane;
-    if (!T.calm && T.gather && e < T.e1) {
+    if (T.gather && e < T.e1) {
       T.src = s.in_cols[e];
       T.val = s.in_vals[e];
     }
   };
   auto first_act = [&](Tile& T) {
     const int e = T.off[0] + grp * kWave + lane;
-    if (!T.calm && T.gather && e < T.e1)
+    if (T.gather && e < T.e1)
       T.a = s.act[T.src * parts + T.part];
+  };
+
+  // a calm tile's window: its record and act words of the window's steps
+  // are cleared, nothing else is read or written
+  auto calm_window = [&](const Tile& T) {
+    const int p0 = T.p0, part = T.part, np = T.np;
+    const int tb = T.tb;
+    const bool live = pl < np && (!s.word_end || t0 < s.word_end[tb >> 6]);
+    if (live && lane < W) {
+      const int pre = neuron_at(p0 + pl);
+      const int wave = (pre * bs + tb) >> 6;
+      if (lane == W - 1) {  // the slot an m_out re-send may reach
+        const int slot =
+            wbase + lane < kRec ? wbase + lane : wbase + lane - kRec;
+        s.rec[(int64_t)pre * pstride + slot * words + (tb >> 6)] = 0ull;
+        if (lists && tl < kWave)  // the part's list: empty
+          *(uint4*)(s.rec_ids +
+                    (((int64_t)pre * parts + part) * kRec + slot) * 16) =
+              make_uint4(0u, 0u, 0u, 0u);
+      }
+      if (s.raster) {
+        int rrow = (int)(t0 % s.raster_len) + lane;
+        rrow = rrow >= s.raster_len ? rrow - s.raster_len : rrow;
+        s.raster[((int64_t)rrow * nblk + (wave >> 2)) * FIS_LINE_WORDS +
+                 (wave & 3)] = 0ull;
+      }
+    }
+    if (tid < np) {
+      const uint64_t nw = T.act_old & ~rot40(wmask, wbase);
+      if (nw != T.act_old)
+        s.act[(int64_t)neuron_at(p0 + tid) * parts + part] = nw;
+    }
   };
 
   // the tile's window
@@ -1326,32 +1359,6 @@ void k_pull(fis_sim s, int W, int ring_steps, int n_tiles, int per_xcd) {
     const int pre = T.pre, idx = T.idx, tb = T.tb;
     const int wave = idx >> 6;
     const uint32_t live_words = T.live;
-    if (T.calm) {
-      const bool live = pl < np && (!s.word_end || t0 < s.word_end[tb >> 6]);
-      if (live && lane < W) {
-        if (lane == W - 1) {  // the slot an m_out re-send may reach
-          const int slot =
-              wbase + lane < kRec ? wbase + lane : wbase + lane - kRec;
-          s.rec[(int64_t)pre * pstride + slot * words + (tb >> 6)] = 0ull;
-          if (lists && tl < kWave)  // the part's list: empty
-            *(uint4*)(s.rec_ids +
-                      (((int64_t)pre * parts + part) * kRec + slot) * 16) =
-                make_uint4(0u, 0u, 0u, 0u);
-        }
-        if (s.raster) {
-          int rrow = (int)(t0 % s.raster_len) + lane;
-          rrow = rrow >= s.raster_len ? rrow - s.raster_len : rrow;
-          s.raster[((int64_t)rrow * nblk + (wave >> 2)) * FIS_LINE_WORDS +
-                   (wave & 3)] = 0ull;
-        }
-      }
-      if (tid < np) {
-        const uint64_t nw = T.act_old & ~rot40(wmask, wbase);
-        if (nw != T.act_old)
-          s.act[(int64_t)neuron_at(p0 + tid) * parts + part] = nw;
-      }
-      return;  // no barrier was passed: uniform over the workgroup
-    }
     // uniform over a wave: a neuron of the tile, in a live 64-trial word
     const bool act = pl < np && (!s.word_end || t0 < s.word_end[tb >> 6]);
 
@@ -1710,7 +1717,12 @@ void k_pull(fis_sim s, int W, int ring_steps, int n_tiles, int per_xcd) {
   // A workgroup that took two tiles in turn, loading the second's state and
   // first in-edges during the first, measured slower (0.80 against 0.75 ms
   // per window at 256 trials: 64 VGPRs and spills).
-  Tile T = begin();
+  Tile T = head();
+  if (T.calm) {
+    calm_window(T);
+    return;  // uniform over the workgroup, and before any barrier
+  }
+  load(T);
   first_edge(T);
   first_act(T);
   run(T);
