@@ -1151,19 +1151,164 @@ __host__ __device__ inline int pull_parts(int bs, int tile) {
   return bs / pull_trials_per_part(bs, tile);
 }
 
+// Gated pull road, first launch of a window: one lane per tile (the tile
+// numbering of k_pull), kPlanTiles tiles per thread.  A tile whose every
+// (neuron, part) is calm and to which no input is due in this window (see
+// k_pull) has its window done here: the act bits of the window's steps are
+// cleared, and its raster rows; nothing else of it is read or written.
+// Its record words and lists of those steps keep stale contents that
+// nothing reads: every reader reads them only under a set act bit, and the
+// one writer that sets act bits outside a window, the m_out re-send
+// (k_pull_resend), sets them only for a neuron-trial that spiked at the
+// last step, which no trial of a calm part did.  Every other tile is
+// appended to the window's work list, which the k_pull launch that follows
+// walks -- so the many calm tiles of a sparse network cost a lane each, not
+// a workgroup each.
+//
+// work[]: [0], [1] the list lengths of the two parities, [2] the parity of
+// the next window, [3] that of the current one, [4 ..] the list.  A window
+// of parity p appends to work[p] (one atomic per workgroup: appends of
+// every wave to one counter serialise); its k_pull reads work[3] and
+// work[p], flips work[2] and zeroes work[1 - p] for the next window -- no
+// counter is read and reset in one launch.
+constexpr int kPlanTiles = 4;
+
+template <int BS, int TILE>
+__global__ __launch_bounds__(kBlock) void k_pull_plan(fis_sim s, int W,
+                                                      int ring_steps,
+                                                      int n_tiles) {
+  constexpr int bs = BS;
+  constexpr int tpn = TILE < BS ? TILE : BS;  // trials of a neuron per tile
+  constexpr int P = TILE / tpn;               // neurons per tile
+  constexpr int parts = BS / tpn;             // tiles per neuron
+  constexpr int words = BS >> 6;              // 64-trial words per neuron
+  constexpr int wpt = tpn >> 6;               // of them in one tile
+  __shared__ int wave_n[kBlock / kWave];      // busy tiles per wave
+  __shared__ int wave_at[kBlock / kWave];     // where each wave's go
+  const int tid = (int)threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int first = (int)blockIdx.x * kBlock * kPlanTiles + tid;
+  const int64_t t0 = now(s);
+  const int par = s.work[2];
+  if (blockIdx.x == 0 && tid == 0) s.work[3] = par;
+  auto neuron_at = [&](int p) { return s.order ? s.order[p] : p; };
+  uint32_t busy = 0;  // bit k: tile first + k * kBlock goes on the list
+#pragma unroll
+  for (int k = 0; k < kPlanTiles; ++k) {
+    const int tile = first + k * kBlock;
+    if (tile >= n_tiles) continue;
+    const int p0 = (tile / parts) * P, part = tile % parts;
+    const int np = min(P, s.n - p0);
+    int pre[P];
+    int64_t latest = -1;
+    bool calm = ring_steps == 0 && s.calm != nullptr;
+    uint64_t act_old[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      pre[q] = neuron_at(p0 + min(q, np - 1));
+      act_old[q] = 0;
+      if (q < np) {
+        const int64_t u = (int64_t)pre[q] * parts + part;
+        const int64_t d = s.due[u];
+        latest = d > latest ? d : latest;
+        if (s.calm && !s.calm[u]) calm = false;
+        act_old[q] = s.act[u];
+      }
+    }
+    if (latest >= t0 || !calm) {
+      busy |= 1u << k;
+      continue;
+    }
+    const int wbase = (int)(t0 % kRec);  // slot of step t0
+    const uint64_t clear = ~rot40((1ull << W) - 1, wbase);
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      if (q >= np) continue;
+      const uint64_t nw = act_old[q] & clear;
+      if (nw != act_old[q]) s.act[(int64_t)pre[q] * parts + part] = nw;
+    }
+    if (s.raster) {  // the window's raster rows of the tile's live words
+      const int w0 = part * wpt;
+      const int nblk = (s.n * bs + kBlock - 1) / kBlock;
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        if (q >= np) continue;
+#pragma unroll
+        for (int j = 0; j < wpt; ++j) {
+          if (s.word_end && !(t0 < s.word_end[w0 + j])) continue;
+          const int wave = pre[q] * words + w0 + j;
+          int rrow = (int)(t0 % s.raster_len);
+          for (int r = 0; r < W; ++r) {
+            s.raster[((int64_t)rrow * nblk + (wave >> 2)) * FIS_LINE_WORDS +
+                     (wave & 3)] = 0ull;
+            rrow = rrow + 1 == s.raster_len ? 0 : rrow + 1;
+          }
+        }
+      }
+    }
+  }
+  // this thread's busy tiles go to list positions base + (busy tiles of
+  // the waves before it) + (of the lanes before it): a prefix of the
+  // counts (<= kPlanTiles) by a ballot per bit, then one atomic per
+  // workgroup
+  const int c = __popc(busy);
+  int excl = 0, total = 0;
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    const uint64_t mb = __ballot((c >> b) & 1);
+    excl += (int)__builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(mb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mb, 0u))
+            << b;
+    total += __popcll(mb) << b;
+  }
+  const int wv = tid >> 6;
+  if (lane == 0) wave_n[wv] = total;
+  __syncthreads();
+  if (tid == 0) {
+    int sum = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / kWave; ++w) {
+      wave_at[w] = sum;
+      sum += wave_n[w];
+    }
+    const int base = sum ? atomicAdd(s.work + par, sum) : 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / kWave; ++w) wave_at[w] += base;
+  }
+  __syncthreads();
+  int at = wave_at[wv] + excl;
+#pragma unroll
+  for (int k = 0; k < kPlanTiles; ++k)
+    if ((busy >> k) & 1u) {
+      // (at most n_tiles entries: the counter starts the window at zero)
+      if (at < n_tiles) s.work[4 + at] = first + k * kBlock;
+      ++at;
+    }
+}
+
+// Workgroups of a gated window (which walk its work list), in waves: the
+// chip holds 8,192 waves of k_pull at once (256 CUs x 32).
+#ifndef FIS_PULL_LIST_WAVES
+#define FIS_PULL_LIST_WAVES 8192
+#endif
+constexpr int kPullListWaves = FIS_PULL_LIST_WAVES;
+
 // Scalar registers capped at 80: measured on MI355X, a workgroup of this
 // kernel is resident one fewer per CU than the compiler's occupancy
 // estimate whenever the kernel uses more than ~80 (96 SGPRs: 7 of 8
 // workgroups, 1.465 ms per window; capped at 80: 8, 1.417 ms).
-template <int WMAX, int BS, int TILE, bool GATE>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_num_sgpr(80)))
-__attribute__((amdgpu_waves_per_eu(8, 8)))
-void k_pull(fis_sim s, int W, int ring_steps, int n_tiles, int per_xcd) {
-  // acc[trial][step]: a lane's WMAX cells are contiguous (cleared and read
-  // back with 16-byte LDS accesses); nothing else lives in LDS
-  static_assert(WMAX % 4 == 0, "acc rows are cleared 4 cells at a time");
-  __shared__ __align__(16) int32_t acc[TILE * WMAX];
+// A kernel-argument address (as an integer) as a pointer into the
+// constant kernel-argument segment, so that loads through it are scalar.
+__device__ __forceinline__ const __attribute__((address_space(4))) void*
+kernarg_ptr(uint64_t a) {
+  return (const __attribute__((address_space(4))) void*)a;
+}
 
+// One tile's window of k_pull (below): `acc` is the workgroup's LDS.
+template <int WMAX, int BS, int TILE, bool GATE>
+__device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
+                                          int tile, int W, int ring_steps,
+                                          int tid) {
   constexpr int bs = BS;                        // trial stride (== s.bstride)
   constexpr int tpn = TILE < BS ? TILE : BS;    // trials of a neuron per tile
   constexpr int P = TILE / tpn;                 // neurons per tile
@@ -1174,16 +1319,9 @@ void k_pull(fis_sim s, int W, int ring_steps, int n_tiles, int per_xcd) {
   // byte; below 128 trials a list is no shorter than the one word)
   constexpr bool LISTS = wpt >= 2 && tpn <= 256;
   const bool lists = LISTS && s.rec_ids != nullptr;
-  const int tid = (int)threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int pl = tid / tpn;                     // position in the tile
   const int tl = tid - pl * tpn;                // trial within the part
-  // XCD-aware tile order: workgroup b is dispatched to XCD b mod 8, and
-  // XCD x takes the consecutive tiles [x * per_xcd, (x + 1) * per_xcd) of
-  // the neuron order, so one XCD's L2 holds the spike records its neurons'
-  // presynaptic partners share
-  const int tile = (int)(blockIdx.x & 7u) * per_xcd + (int)(blockIdx.x >> 3);
-  if (tile >= n_tiles) return;  // whole workgroup, before any barrier
   const int64_t t0 = now(s);
   const int rbase = (int)((t0 + kRec - 20) % kRec);  // slot of step t0-20
   const int wbase = (int)(t0 % kRec);                // slot of step t0
@@ -1207,7 +1345,7 @@ void k_pull(fis_sim s, int W, int ring_steps, int n_tiles, int per_xcd) {
   // What a tile needs from memory before its gather: loaded ahead.
   struct Tile {
     int p0, part, w0, np;
-    bool gather, calm;
+    bool gather;
     uint32_t calm_old;    // bit q: neuron q of the tile was calm
     uint32_t live;        // bit j: the tile's word j is still simulated
     uint64_t act_old;     // act word of neuron `tid` (tid < np)
@@ -1221,7 +1359,7 @@ void k_pull(fis_sim s, int W, int ring_steps, int n_tiles, int per_xcd) {
     uint64_t a;           // and its act word
   };
   // the tile, whether it is calm, and its act words
-  auto head = [&]() {
+  auto head = [&](int tile) {
     Tile T;
     T.p0 = (tile / parts) * P;   // first position of the tile
     T.part = tile % parts;       // which part of its neuron(s)
@@ -1248,19 +1386,15 @@ void k_pull(fis_sim s, int W, int ring_steps, int n_tiles, int per_xcd) {
     // address), so that nothing waits for it before the end.
     T.act_old = s.act[(int64_t)neuron_at(T.p0 + opaque_vgpr(min(tid, T.np - 1))) *
                           parts + T.part];
-    // calm: every neuron of the tile sat at a fixed point through the last
-    // window, and no input, kick or carried-over ring input reaches it in
-    // this one -- the window changes nothing but the record and act words,
-    // and its state is not read (uniform)
-    T.calm = false;
+    // which neurons of the tile were calm (a tile with no input due whose
+    // every neuron was calm had its window done by k_pull_plan and is not
+    // on the work list)
     T.calm_old = 0;
     if (GATE && s.calm) {
 #pragma unroll
       for (int q = 0; q < P; ++q)
         if (q < T.np && s.calm[(int64_t)neuron_at(T.p0 + q) * parts + T.part])
           T.calm_old |= 1u << q;
-      T.calm = !T.gather && ring_steps == 0 &&
-               T.calm_old == (1u << T.np) - 1u;
     }
     T.tb = (T.w0 << 6) + tl;
     return T;
@@ -1319,38 +1453,6 @@ void k_pull(fis_sim s, int W, int ring_steps, int n_tiles, int per_xcd) {
     const int e = T.off[0] + grp * kWave + lane;
     if (T.gather && e < T.e1)
       T.a = s.act[T.src * parts + T.part];
-  };
-
-  // a calm tile's window: its record and act words of the window's steps
-  // are cleared, nothing else is read or written
-  auto calm_window = [&](const Tile& T) {
-    const int p0 = T.p0, part = T.part, np = T.np;
-    const int tb = T.tb;
-    const bool live = pl < np && (!s.word_end || t0 < s.word_end[tb >> 6]);
-    if (live && lane < W) {
-      const int pre = neuron_at(p0 + pl);
-      const int wave = (pre * bs + tb) >> 6;
-      if (lane == W - 1) {  // the slot an m_out re-send may reach
-        const int slot =
-            wbase + lane < kRec ? wbase + lane : wbase + lane - kRec;
-        s.rec[(int64_t)pre * pstride + slot * words + (tb >> 6)] = 0ull;
-        if (lists && tl < kWave)  // the part's list: empty
-          *(uint4*)(s.rec_ids +
-                    (((int64_t)pre * parts + part) * kRec + slot) * 16) =
-              make_uint4(0u, 0u, 0u, 0u);
-      }
-      if (s.raster) {
-        int rrow = (int)(t0 % s.raster_len) + lane;
-        rrow = rrow >= s.raster_len ? rrow - s.raster_len : rrow;
-        s.raster[((int64_t)rrow * nblk + (wave >> 2)) * FIS_LINE_WORDS +
-                 (wave & 3)] = 0ull;
-      }
-    }
-    if (tid < np) {
-      const uint64_t nw = T.act_old & ~rot40(wmask, wbase);
-      if (nw != T.act_old)
-        s.act[(int64_t)neuron_at(p0 + tid) * parts + part] = nw;
-    }
   };
 
   // the tile's window
@@ -1694,10 +1796,12 @@ void k_pull(fis_sim s, int W, int ring_steps, int n_tiles, int per_xcd) {
 
     // 4. input gating: a neuron of the tile that sent anything (in this
     // part) stamps its targets with the last step its spikes can arrive
-    // at.  Every workgroup of a launch stores the same stamp, and stamps
-    // only grow from launch to launch, so plain stores suffice.
+    // at: an atomic max that returns nothing (stamps only grow), so that
+    // a hub's thousands of targets cost a few rounds of target loads, each
+    // followed by stores nothing waits for.
     if (GATE) {
-      const int64_t stamp = t0 + W - 1 + kDelay;
+      const uint64_t stamp = (uint64_t)(t0 + W - 1 + kDelay);
+      constexpr int U = 4;  // targets per thread loaded at once
       for (int q = 0; q < np; ++q) {
         uint32_t sent = 0;
 #pragma unroll
@@ -1706,26 +1810,78 @@ void k_pull(fis_sim s, int W, int ring_steps, int n_tiles, int per_xcd) {
         if (!sent) continue;  // uniform: every thread reads the same cells
         const int src = neuron_at(p0 + q);
         const int64_t b1 = s.offsets[src + 1];
-        for (int64_t k = s.offsets[src] + tid; k < b1; k += TILE) {
-          int64_t* d = s.due + (int64_t)s.cols[k] * parts + part;
-          if (*d < stamp) *d = stamp;  // a target stamped is not rewritten
+        for (int64_t k = s.offsets[src] + tid; k < b1; k += U * TILE) {
+          int32_t c[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            c[u] = k + u * TILE < b1 ? s.cols[k + u * TILE] : -1;
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            if (c[u] >= 0)
+              __hip_atomic_fetch_max(
+                  (unsigned long long*)(s.due + (int64_t)c[u] * parts + part),
+                  (unsigned long long)stamp, __ATOMIC_RELAXED,
+                  __HIP_MEMORY_SCOPE_AGENT);
         }
       }
     }
   };
 
-  // A workgroup that took two tiles in turn, loading the second's state and
-  // first in-edges during the first, measured slower (0.80 against 0.75 ms
-  // per window at 256 trials: 64 VGPRs and spills).
-  Tile T = head();
-  if (T.calm) {
-    calm_window(T);
-    return;  // uniform over the workgroup, and before any barrier
-  }
+  Tile T = head(tile);
   load(T);
   first_edge(T);
   first_act(T);
   run(T);
+}
+
+template <int WMAX, int BS, int TILE, bool GATE>
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_num_sgpr(80)))
+__attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_pull(fis_sim s, int W, int ring_steps, int n_tiles, int per_xcd) {
+  // acc[trial][step]: a lane's WMAX cells are contiguous (cleared and read
+  // back with 16-byte LDS accesses); nothing else lives in LDS
+  static_assert(WMAX % 4 == 0, "acc rows are cleared 4 cells at a time");
+  __shared__ __align__(16) int32_t acc[TILE * WMAX];
+  if constexpr (!GATE) {
+    // XCD-aware tile order: workgroup b is dispatched to XCD b mod 8, and
+    // XCD x takes the consecutive tiles [x * per_xcd, (x + 1) * per_xcd) of
+    // the neuron order, so one XCD's L2 holds the spike records its
+    // neurons' presynaptic partners share.  (A workgroup that took two
+    // tiles in turn, loading the second's state and first in-edges during
+    // the first, measured slower: 0.80 against 0.75 ms per window at 256
+    // trials, 64 VGPRs and spills.)
+    const int tile = (int)(blockIdx.x & 7u) * per_xcd + (int)(blockIdx.x >> 3);
+    if (tile >= n_tiles) return;  // whole workgroup, before any barrier
+    pull_tile<WMAX, BS, TILE, GATE>(s, acc, tile, W, ring_steps,
+                                    (int)threadIdx.x);
+  } else {
+    // the window's work list (k_pull_plan), in 8 runs of consecutive
+    // entries: run x is walked by the workgroups of XCD x, each taking
+    // every (grid / 8)-th entry
+    const int par = s.work[3];
+    const int cnt = min(s.work[par], n_tiles);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // the next window's counter
+      s.work[2] = 1 - par;
+      s.work[1 - par] = 0;
+    }
+    const int nk = (int)(gridDim.x >> 3);
+    const int per = (cnt + 7) >> 3;
+    const int lo = (int)(blockIdx.x & 7u) * per;
+    const int hi = min(cnt, lo + per);
+    for (int i = lo + (int)(blockIdx.x >> 3); i < hi; i += nk) {
+      // Each tile starts from values the compiler cannot see through: the
+      // arguments read again from the kernel-argument segment (scalar
+      // loads), the thread number and the window's sizes.  Nothing of one
+      // tile's window is then hoisted out of the loop and held in
+      // registers across it (it would spill).
+      uint64_t ka = (uint64_t)__builtin_amdgcn_kernarg_segment_ptr();
+      int w = W, rs = ring_steps, tid = (int)threadIdx.x;
+      asm volatile("" : "+s"(ka), "+s"(w), "+s"(rs), "+v"(tid));
+      const fis_sim& sk = *(const fis_sim*)kernarg_ptr(ka);
+      pull_tile<WMAX, BS, TILE, GATE>(sk, acc, sk.work[4 + i], w, rs, tid);
+      __syncthreads();  // the next tile clears the accumulators
+    }
+  }
 }
 
 // clock += W after a window (one thread; every workgroup of the window has
@@ -1946,6 +2102,7 @@ int check_pull(const fis_sim* s) {
     return (int)hipErrorInvalidValue;
   if (!s->in_offsets || !s->in_cols || !s->in_vals || !s->rec || !s->act)
     return (int)hipErrorInvalidValue;
+  if (s->due && !s->work) return (int)hipErrorInvalidValue;
   if (s->pull_tile != 0 && s->pull_tile != 128 && s->pull_tile != 256 &&
       s->pull_tile != 512)
     return (int)hipErrorInvalidValue;
@@ -2040,6 +2197,7 @@ int64_t fis_struct_layout(int what) {
     case 12: return (int64_t)offsetof(fis_sim, due);
     case 13: return (int64_t)offsetof(fis_sim, calm);
     case 14: return (int64_t)offsetof(fis_sim, rec_ids);
+    case 15: return (int64_t)offsetof(fis_sim, work);
     default: return -1;
   }
 }
@@ -2201,11 +2359,13 @@ int fis_pull_window(const fis_sim* s, int W, int ring_steps, void* stream) {
   const int tile = pull_tile_of(*s);
   const bool gate = s->due != nullptr;
   auto kern = k_pull<FIS_MAX_WINDOW, 256, 256, false>;
+  auto plan = k_pull_plan<256, 256>;
   switch (s->bstride * 4 + tile / 128) {  // tile / 128 is 1, 2 or 4
 #define FIS_PULL_CASE(B, T)                                    \
   case B * 4 + T / 128:                                        \
     kern = gate ? k_pull<FIS_MAX_WINDOW, B, T, true>           \
                 : k_pull<FIS_MAX_WINDOW, B, T, false>;         \
+    plan = k_pull_plan<B, T>;                                  \
     break;
     FIS_PULL_CASE(64, 128) FIS_PULL_CASE(64, 256) FIS_PULL_CASE(64, 512)
     FIS_PULL_CASE(128, 128) FIS_PULL_CASE(128, 256) FIS_PULL_CASE(128, 512)
@@ -2223,8 +2383,24 @@ int fis_pull_window(const fis_sim* s, int W, int ring_steps, void* stream) {
   // whole neurons per XCD: the parts of one neuron write the same record
   // lines, and share one L2
   const int per_xcd = ((n_tiles + 7) / 8 + parts - 1) / parts * parts;
-  hipLaunchKernelGGL(kern, dim3((unsigned)(8 * per_xcd)), dim3(tile), 0,
-                     (hipStream_t)stream, *s, W, ring_steps, n_tiles, per_xcd);
+  if (gate) {
+    // the work list first, then kPullListWaves waves' worth of
+    // workgroups (a multiple of 8) walk it
+    hipLaunchKernelGGL(plan, dim3(grid_for((n_tiles + kPlanTiles - 1) /
+                                           kPlanTiles)),
+                       dim3(kBlock), 0, (hipStream_t)stream, *s, W,
+                       ring_steps, n_tiles);
+    const int most = kPullListWaves * kWave / tile;
+    const int up8 = (n_tiles + 7) / 8 * 8;
+    const int grid = most < up8 ? most : up8;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(tile), 0,
+                       (hipStream_t)stream, *s, W, ring_steps, n_tiles,
+                       per_xcd);
+  } else {
+    hipLaunchKernelGGL(kern, dim3((unsigned)(8 * per_xcd)), dim3(tile), 0,
+                       (hipStream_t)stream, *s, W, ring_steps, n_tiles,
+                       per_xcd);
+  }
   if (!s->defer_tick)
     hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, (hipStream_t)stream,
                        s->clock, W);

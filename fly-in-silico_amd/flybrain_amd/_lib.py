@@ -20,7 +20,7 @@ CSRC = PKG.parent / "csrc"
 SOURCES = [CSRC / "lif.hip"]
 HEADER = ROOT / "include" / "fis.h"
 LIB_PATH = PKG / "libfis.so"
-ABI_VERSION = 14
+ABI_VERSION = 15
 
 MAX_WINDOW = 20
 SCRATCH_BYTES = 64 << 20  # FIS_SCRATCH_BYTES: fis_window_noalloc's staging
@@ -106,6 +106,7 @@ class FisSim(ctypes.Structure):
         ("pull_tile", ctypes.c_int32),
         ("clock_ahead", ctypes.c_int32), ("defer_tick", ctypes.c_int32),
         ("drive_dense", P), ("due", P), ("calm", P), ("rec_ids", P),
+        ("work", P),
     ]
 
 
@@ -175,7 +176,7 @@ def lib():
             8: FisSim.t_origin.offset, 9: FisSim.pull_tile.offset,
             10: FisSim.clock_ahead.offset, 11: FisSim.drive_dense.offset,
             12: FisSim.due.offset, 13: FisSim.calm.offset,
-            14: FisSim.rec_ids.offset}
+            14: FisSim.rec_ids.offset, 15: FisSim.work.offset}
     for k, v in want.items():
         got = L.fis_struct_layout(k)
         if got != v:

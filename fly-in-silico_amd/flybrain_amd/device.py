@@ -248,9 +248,16 @@ class DeviceSim:
             self.calm = (torch.zeros((n * parts + 3) // 4 * 4,
                                      dtype=torch.uint8, **z)[:n * parts]
                          if gate else None)
+            # a gated window's work list: two counters and two parities,
+            # then the tiles it runs (zero at the start)
+            per = max(1, self.pull_tile // bs)
+            tiles = (n + per - 1) // per * parts
+            self.work = (torch.zeros(4 + tiles, dtype=torch.int32, **z)
+                         if gate else None)
         else:
             self.order = self.in_offsets = self.in_cols = self.in_vals = None
             self.rec = self.act = self.due = self.calm = None
+            self.work = None
             self.rec_ids = None
         self.gate = self.due is not None
         self.t = 0  # host mirror of the device clock
@@ -298,7 +305,7 @@ class DeviceSim:
         s.raster_len = self.raster_len
         for name in ("in_offsets", "in_cols", "in_vals", "rec", "act",
                      "order", "word_end", "drive_group", "drive_rate",
-                     "due", "calm", "rec_ids"):
+                     "due", "calm", "rec_ids", "work"):
             t = getattr(self, name, None)
             setattr(s, name, t.data_ptr() if t is not None else None)
         dims = getattr(self, "_drive_dims", (0, 0, 0))

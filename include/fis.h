@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define FIS_ABI_VERSION 14
+#define FIS_ABI_VERSION 15
 #define FIS_MAX_WINDOW 20   /* steps per window: delay horizon 19 + 1     */
 #define FIS_RING_LEN 40     /* int32 ring slots, two delays: the spikes of
                                step u are summed into slot (u + 21) mod 40,
@@ -226,6 +226,13 @@ typedef struct {
    * (empty lists, matching zero words) at the start; a caller that writes
    * rec itself zeroes the lists with it or fills them with 0xFF. */
   uint8_t* rec_ids;
+  /* Pull road, required with `due`: [4 + tiles] int32, zero at the start,
+   * tiles = ceil(n / max(1, pull_tile / bstride)) * max(1, bstride /
+   * pull_tile).  A gated window first lists the tiles it has to run (the
+   * others -- calm, no input due -- are done by that first launch), then
+   * walks the list: work[0..3] are the list's counters and parities (kept
+   * by the library from window to window), work[4 ..] the list. */
+  int32_t* work;
 } fis_sim;
 
 /* Drive table entry: neuron-trial `idx` is driven at rate
@@ -381,8 +388,8 @@ int fis_flag_pairs(const fis_sim* s, const int32_t* idx, int n,
 /* sizeof / offsetof of the structs above (0: sizeof fis_sim, 1: offsets,
  * 2: w_scale, 3: refrac_steps, 4: raster_len, 5: sizeof fis_event,
  * 6: sizeof fis_drive_entry, 7: in_offsets, 8: t_origin, 9: pull_tile,
- * 10: clock_ahead, 11: drive_dense, 12: due, 13: calm, 14: rec_ids), so a
- * binding can check its mirror. */
+ * 10: clock_ahead, 11: drive_dense, 12: due, 13: calm, 14: rec_ids,
+ * 15: work), so a binding can check its mirror. */
 int64_t fis_struct_layout(int what);
 
 /* Number of update waves (event segments) for a given n and bstride. */

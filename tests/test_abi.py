@@ -43,7 +43,7 @@ def test_versions_and_layout(lib):
                         (4, "raster_len"), (7, "in_offsets"), (8, "t_origin"),
                         (9, "pull_tile"), (10, "clock_ahead"),
                         (11, "drive_dense"), (12, "due"), (13, "calm"),
-                        (14, "rec_ids")):
+                        (14, "rec_ids"), (15, "work")):
         assert lib.fis_struct_layout(what) == getattr(L.FisSim, field).offset
 
 
