@@ -1286,10 +1286,13 @@ __global__ __launch_bounds__(kBlock) void k_pull_plan(fis_sim s, int W,
     }
 }
 
-// Workgroups of a gated window (which walk its work list), in waves: the
-// chip holds 8,192 waves of k_pull at once (256 CUs x 32).
+// Workgroups of a gated window (which walk its work list), in waves: twice
+// the 8,192 waves of k_pull the chip holds at once (256 CUs x 32), so that
+// workgroups that finish early are followed by others.  Measured at 256
+// trials, 138,639 neurons, plain drive: 2.19 us per step against 2.26 with
+// one resident grid.
 #ifndef FIS_PULL_LIST_WAVES
-#define FIS_PULL_LIST_WAVES 8192
+#define FIS_PULL_LIST_WAVES 16384
 #endif
 constexpr int kPullListWaves = FIS_PULL_LIST_WAVES;
 
@@ -1858,8 +1861,10 @@ void k_pull(fis_sim s, int W, int ring_steps, int n_tiles, int per_xcd) {
     // the window's work list (k_pull_plan), in 8 runs of consecutive
     // entries: run x is walked by the workgroups of XCD x, each taking
     // every (grid / 8)-th entry
-    const int par = s.work[3];
-    const int cnt = min(s.work[par], n_tiles);
+    // (both counters and the parity read at once: one round trip)
+    const int4 head4 = *(const int4*)s.work;
+    const int par = head4.w;
+    const int cnt = min(par ? head4.y : head4.x, n_tiles);
     if (blockIdx.x == 0 && threadIdx.x == 0) {  // the next window's counter
       s.work[2] = 1 - par;
       s.work[1 - par] = 0;
@@ -1868,7 +1873,12 @@ void k_pull(fis_sim s, int W, int ring_steps, int n_tiles, int per_xcd) {
     const int per = (cnt + 7) >> 3;
     const int lo = (int)(blockIdx.x & 7u) * per;
     const int hi = min(cnt, lo + per);
-    for (int i = lo + (int)(blockIdx.x >> 3); i < hi; i += nk) {
+    int i = lo + (int)(blockIdx.x >> 3);
+    int next = i < hi ? s.work[4 + i] : 0;
+    for (; i < hi; i += nk) {
+      const int tile = next;
+      // the entry after this one, read while this tile runs
+      next = i + nk < hi ? s.work[4 + i + nk] : 0;
       // Each tile starts from values the compiler cannot see through: the
       // arguments read again from the kernel-argument segment (scalar
       // loads), the thread number and the window's sizes.  Nothing of one
@@ -1878,7 +1888,7 @@ void k_pull(fis_sim s, int W, int ring_steps, int n_tiles, int per_xcd) {
       int w = W, rs = ring_steps, tid = (int)threadIdx.x;
       asm volatile("" : "+s"(ka), "+s"(w), "+s"(rs), "+v"(tid));
       const fis_sim& sk = *(const fis_sim*)kernarg_ptr(ka);
-      pull_tile<WMAX, BS, TILE, GATE>(sk, acc, sk.work[4 + i], w, rs, tid);
+      pull_tile<WMAX, BS, TILE, GATE>(sk, acc, tile, w, rs, tid);
       __syncthreads();  // the next tile clears the accumulators
     }
   }
@@ -2102,7 +2112,8 @@ int check_pull(const fis_sim* s) {
     return (int)hipErrorInvalidValue;
   if (!s->in_offsets || !s->in_cols || !s->in_vals || !s->rec || !s->act)
     return (int)hipErrorInvalidValue;
-  if (s->due && !s->work) return (int)hipErrorInvalidValue;
+  if (s->due && (!s->work || ((uintptr_t)s->work & 15u) != 0))
+    return (int)hipErrorInvalidValue;
   if (s->pull_tile != 0 && s->pull_tile != 128 && s->pull_tile != 256 &&
       s->pull_tile != 512)
     return (int)hipErrorInvalidValue;

@@ -231,7 +231,8 @@ typedef struct {
    * pull_tile).  A gated window first lists the tiles it has to run (the
    * others -- calm, no input due -- are done by that first launch), then
    * walks the list: work[0..3] are the list's counters and parities (kept
-   * by the library from window to window), work[4 ..] the list. */
+   * by the library from window to window), work[4 ..] the list.  16-byte
+   * aligned. */
   int32_t* work;
 } fis_sim;
 
