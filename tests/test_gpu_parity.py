@@ -513,7 +513,9 @@ def test_calm_tiles_wake_up_exactly(batch, tile):
     (kicks reach calm neurons: the drive launch ends their calm) and
     its spikes reach calm targets (input due ends the skip).  Raster and
     state equal the oracle throughout; calm entries were set and later
-    cleared."""
+    cleared; every record list under a set act bit names the trials of its
+    record words (calm tiles leave their own stale, under clear act
+    bits)."""
     conn = C.synthetic(640, None, mean_out=8, count_exp=1.2, n_groups=4,
                        seed=31)
     n = conn.n
@@ -528,6 +530,8 @@ def test_calm_tiles_wake_up_exactly(batch, tile):
     net = DeviceNet(conn)
     sim = DeviceSim(net, batch, seeds=seeds, raster_len=40, road="pull",
                     pull_tile=tile, gate=True)
+    # parts of 128 and 256 trials also keep record lists
+    lists = min(sim.bstride, tile) // 64 >= 2
     ne = np.concatenate(drv)
     tr = np.concatenate([np.full(len(d), b) for b, d in enumerate(drv)])
     sim.exempt_pairs(ne, tr)
@@ -556,9 +560,14 @@ def test_calm_tiles_wake_up_exactly(batch, tile):
                 "raster differs at step %d" % t
         if a + 20 == 300:
             calm_seen = int(sim.calm.sum().item())
+            if lists:  # calm tiles keep stale words under clear act bits
+                check_record_lists(sim)
     compare_state(sim, ora, "end")
     assert calm_seen > 0, "no (neuron, part) went calm before the drive"
     assert int(sim.calm.sum().item()) < calm_seen
+    if lists:
+        checked, _ = check_record_lists(sim)
+        assert checked > 0
 
 
 def check_record_lists(sim):
