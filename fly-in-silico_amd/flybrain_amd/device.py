@@ -872,9 +872,13 @@ class DriveTable:
         if np.unique(idx).size != idx.size:
             raise ValueError("a neuron-trial appears twice in the drive table")
         check_rates_hz(rt, "drive table")
+        # the device copy in neuron-trial order: a wave's entries then set
+        # kick words side by side (a few lines per wave) instead of one line
+        # each (the entries are independent, so their order is free)
+        order = np.argsort(idx, kind="stable")
         ent = np.empty((idx.size, 2), np.uint32)
-        ent[:, 0] = idx
-        ent[:, 1] = gr
+        ent[:, 0] = idx[order]
+        ent[:, 1] = gr[order]
         self.idx_host, self.group_host = idx, gr
         self.rate_host = np.ascontiguousarray(rt)
         self.n_entries = int(idx.size)

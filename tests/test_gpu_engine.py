@@ -253,26 +253,36 @@ def test_tuned_runner_matches_default(tmp_path, monkeypatch):
     np.testing.assert_array_equal(tuned.counts(), base.counts())
 
 
-@pytest.mark.parametrize("batch,road,fused", [(1, "push", True),
-                                              (3, "push", False),
-                                              (64, "push", False),
-                                              (64, "pull", False),
-                                              (256, "pull", False)])
-def test_graph_replays_equal_eager_windows(batch, road, fused):
+@pytest.mark.parametrize("batch,road,fused,gate,sparse",
+                         [(1, "push", True, False, False),
+                          (3, "push", False, False, False),
+                          (64, "push", False, False, False),
+                          (64, "pull", False, False, False),
+                          (256, "pull", False, False, False),
+                          (64, "pull", False, True, False),
+                          (256, "pull", False, True, True)])
+def test_graph_replays_equal_eager_windows(batch, road, fused, gate, sparse):
     """A captured graph of several windows leaves the device clock alone
     until its one closing tick (each window reads the clock plus the steps
     launched before it).  Replay after replay, with eager windows between,
     it gives the counts, state and clock of the same windows run one by
-    one; the drive table's rates change every 7 steps inside a replay."""
+    one; the drive table's rates change every 7 steps inside a replay.
+    Gated, each window is a planning launch and a walk of its work list,
+    whose counters alternate from window to window inside the graph and
+    out of it; the sparse drive leaves most tiles calm."""
     from flybrain_amd import workloads as WL
     from flybrain_amd.batch import BatchRunner
 
-    conn = C.synthetic(1024, None, mean_out=50, count_exp=1.2, n_groups=4,
-                       seed=37)
+    # (sparse: few out-edges, so most neurons are never reached and settle)
+    conn = C.synthetic(1024, None, mean_out=6 if sparse else 50,
+                       count_exp=1.2, n_groups=4, seed=37)
     net = DeviceNet(conn)
     if batch == 1:
         wl = WL.plain_b1(conn.n, seed=5, steps=1000, n_driven=60,
                          rate_hz=400.0)
+    elif sparse:
+        wl = WL.plain_replicated(conn.n, seed=5, batch=batch, steps=1000,
+                                 n_driven=4, rate_hz=300.0)
     else:
         wl = WL.trials_b64(conn.n, seed=5, batch=batch, steps=1000)
     if not fused:  # rates that change every 7 steps, also inside windows
@@ -281,9 +291,12 @@ def test_graph_replays_equal_eager_windows(batch, road, fused):
         wl.rate_hz = rng.uniform(0.0, 600.0, size=(150, n_groups)).astype(
             np.float32)
         wl.seg_steps = 7
-    graph = BatchRunner(net, wl, counts=True, road=road, fused_drive=fused)
-    eager = BatchRunner(net, wl, counts=True, road=road, fused_drive=fused)
+    graph = BatchRunner(net, wl, counts=True, road=road, fused_drive=fused,
+                        gate=gate)
+    eager = BatchRunner(net, wl, counts=True, road=road, fused_drive=fused,
+                        gate=gate)
     assert graph.fused == eager.fused == fused
+    assert graph.sim.gate == eager.sim.gate == gate
     graph.capture(3)
     for rep in range(3):
         graph.advance(60)  # one replay of three 20-step windows
@@ -298,6 +311,8 @@ def test_graph_replays_equal_eager_windows(batch, road, fused):
         graph.advance(20, use_graph=False)  # an eager window between
         eager.advance(20, use_graph=False)
     assert eager.counts().sum() > 0
+    if sparse:  # tiles went calm: the planning launch did their windows
+        assert int(graph.sim.calm.sum().item()) > 0
 
 
 @pytest.mark.parametrize("road", ["push", "pull"])

@@ -3,8 +3,9 @@
 * The plain workload (20 neurons at 150 Hz, B = 1) from a recorded Poisson
   tape against the oracle: raster identical over the first 100 ms, v
   bit-exact at 100 ms, per-neuron spike counts over 1 s identical.
-* Configs 3 and 4 replayed from captured graphs against the oracle, trial
-  by trial, bit for bit; the Philox drive's rates against a numpy
+* Configs 3 and 4, and the plain drive in 256 trials on the gated road,
+  replayed from captured graphs against the oracle, trial by trial, bit for
+  bit; the Philox drive's rates against a numpy
   Bernoulli oracle (Pearson >= 0.99); serving on the full connectome.
 * Batched trials: trials of the B = 256 sensory batch (pull road) and of
   the B = 64 trial batch give exactly the raster of the same trial run
@@ -83,20 +84,25 @@ def _oracle_for_trial(conn, wl, b):
     return ora, wl.drive_neuron[d], wl.drive_group[d]
 
 
-@pytest.mark.parametrize("maker,trials", [(WL.sensory_b256, (0, 97, 255)),
-                                          (WL.trials_b64, (3, 63))])
-def test_batched_graph_replay_matches_oracle_full_size(full, maker, trials):
-    """Configs 3 and 4 at full size as the bench runs them: the batch's
-    windows replayed from a captured HIP graph (5 windows per replay) for
-    1,000 steps.  For the named trials every step's raster and the final
-    v and g equal, bit for bit, the oracle running that trial alone with
-    its seed, rate schedule (10 ms segments for the sensory batch),
-    refractory exemptions and m_out mask."""
+@pytest.mark.parametrize("maker,trials,gate",
+                         [(WL.sensory_b256, (0, 97, 255), False),
+                          (WL.trials_b64, (3, 63), False),
+                          (WL.plain_replicated, (0, 255), True)])
+def test_batched_graph_replay_matches_oracle_full_size(full, maker, trials,
+                                                       gate):
+    """Configs 3 and 4, and the plain drive in 256 trials on the gated road
+    (planning launch + work list), at full size as the bench runs them: the
+    batch's windows replayed from a captured HIP graph (5 windows per
+    replay) for 1,000 steps.  For the named trials every step's raster and
+    the final v and g equal, bit for bit, the oracle running that trial
+    alone with its seed, rate schedule (10 ms segments for the sensory
+    batch), refractory exemptions and m_out mask."""
     conn, net = full
     steps, block = 1000, 100
     wl = maker(conn.n, seed=0, steps=steps)
-    run = BatchRunner(net, wl, counts=True, raster_len=block)
-    assert run.sim.road == "pull"
+    run = BatchRunner(net, wl, counts=True, raster_len=block, road="pull",
+                      gate=gate)
+    assert run.sim.road == "pull" and run.sim.gate == gate
     run.capture(block // run.window)
     oras = {b: _oracle_for_trial(conn, wl, b) for b in trials}
     rates = {b: np.zeros((1, conn.n), np.float32) for b in trials}
@@ -117,6 +123,8 @@ def test_batched_graph_replay_matches_oracle_full_size(full, maker, trials):
                 spikes[b] += int(sp.sum())
     v = run.sim.view(run.sim.v).cpu().numpy()
     g = run.sim.view(run.sim.g).cpu().numpy()
+    if gate:  # most of the network sat calm: the plan did those windows
+        assert int(run.sim.calm.sum().item()) > conn.n // 2
     for b, (ora, _, _) in oras.items():
         assert spikes[b] > 0
         np.testing.assert_array_equal(v[b].view(np.uint32),
