@@ -427,6 +427,34 @@ class DeviceSim:
         self._drive_dims = (table.n_groups, table.n_seg, table.seg_steps)
         self._sync_struct()
 
+    def fuse_drive_groups(self, groups, rate_hz, n_groups, n_seg=1,
+                          seg_steps=1 << 30):
+        """Push road: the update launches draw kicks for every
+        neuron-trial idx with ``groups[idx] >= 0`` at rate
+        ``rate_hz[min(t // seg_steps, n_seg - 1), groups[idx]]`` (t the
+        trial's own step) -- the bits a drive table of those (idx, group)
+        entries would set.  ``groups`` (int16 [n * bstride]) and
+        ``rate_hz`` (float32 [n_seg * n_groups]) are device tensors the
+        caller keeps and may change between windows."""
+        if self.road != "push":
+            raise ValueError("the pull road draws a drive table in its own "
+                             "launch (drive_table), not inside the window")
+        if not 0 < n_groups <= 32767 or n_seg < 1 or seg_steps < 1:
+            raise ValueError("fused drive: 1..32767 groups, n_seg and "
+                             "seg_steps >= 1")
+        if (groups.dtype != torch.int16 or groups.numel() != self.nb
+                or rate_hz.dtype != torch.float32
+                or rate_hz.numel() != n_groups * n_seg
+                or groups.device != self.device
+                or rate_hz.device != self.device):
+            raise ValueError("fused drive arrays: int16 [n * bstride] groups "
+                             "and float32 [n_seg * n_groups] rates on %s"
+                             % (self.device,))
+        self.drive_group = groups
+        self.drive_rate = rate_hz
+        self._drive_dims = (int(n_groups), int(n_seg), int(seg_steps))
+        self._sync_struct()
+
     # ------------------------------------------------------------ slots
     def restart_trials(self, trials, seeds, t_end, out_neurons=(),
                        out_trials=(), free_neurons=(), free_trials=()):

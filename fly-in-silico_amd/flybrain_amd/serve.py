@@ -31,7 +31,9 @@ the result truncated to what ran, lab/flybrain/jobs.py:348-412):
   * every request is checked when it is submitted (steps, drive and
     silenced indices, drive size, rate); a bad one comes back as a result
     with ``error`` set and never reaches a slot or the device;
-  * ``cancel(rid)`` takes effect at the next window boundary: a queued
+  * ``cancel(rid)`` takes effect at the next window boundary the serving
+    loop reaches (it launches up to ``batch_windows`` windows at a time
+    while no request ends and no command waits): a queued
     request is dropped, a running one is read out with the counts of the
     steps it ran (``cancelled``, ``steps_done``) and its slot freed;
   * ``evict(rid)`` saves a running request's trial state (its column of
@@ -234,13 +236,17 @@ def bytes_per_trial(n, road):
 
 
 class _Shard:
-    """One DeviceSim of ``size`` slots and its drive table.
+    """One DeviceSim of ``size`` slots and its drive.
 
-    The drive table has a fixed layout: slot s owns entries
+    Slot s draws its kicks at rate ``_rate[s]`` (``_rate[size]`` = 0).  On
+    the push road the update launch draws them itself (fused drive: a
+    group index per neuron-trial, the slot's for its driven neurons, -1
+    elsewhere; one launch per window).  On the pull road a drive table
+    with a fixed layout feeds a drive launch: slot s owns entries
     [s * max_drive, (s + 1) * max_drive) and rate group s; group ``size``
     has rate 0 and pads the unused entries.  Admission writes one slot's
-    range, retirement zeroes its rate, and the table's pointers and size
-    never change (a captured graph stays valid).
+    part, retirement zeroes its rate, and no pointer or size ever changes
+    (a captured graph stays valid).
     """
 
     def __init__(self, net, size, road, max_drive):
@@ -263,49 +269,100 @@ class _Shard:
         self._rate = torch.zeros(self.size + 1, dtype=torch.float32,
                                  device=dev)
         self.live = 0
+        self.fused = self.sim.road == "push"
+        if self.fused:
+            self._groups = torch.full((self.sim.nb,), -1, dtype=torch.int16,
+                                      device=dev)
+            self._slot_idx = [None] * self.size  # driven neuron-trials
+            self.sim.fuse_drive_groups(self._groups, self._rate,
+                                       self.size + 1)
+        self._clock = None  # the open deferred-clock run, if any
+        # the shard's own stream (default priority), after its set-up
+        self.stream = torch.cuda.Stream(device=dev)
+        self.stream.wait_stream(torch.cuda.current_stream(dev))
+
+    def on(self):
+        """Run on the shard's stream: the work of different shards overlaps
+        on the GPU, a shard's own stays in order."""
+        return torch.cuda.stream(self.stream)
 
     def set_slot_drive(self, slot, drive, rate_hz):
-        md = self.max_drive
-        r0 = slot * md
-        rows = self._ent_host[r0:r0 + md]
-        rows[:, 0] = slot
-        rows[:, 1] = self.size
-        rows[:drive.size, 0] = drive.astype(np.int64) * self.sim.bstride + slot
-        rows[:drive.size, 1] = slot
-        self._entries[2 * r0:2 * (r0 + md)].copy_(
-            torch.from_numpy(rows.view(np.int32).reshape(-1)))
+        if self.fused:
+            old = self._slot_idx[slot]
+            if old is not None:
+                self._groups[old] = -1
+            idx = torch.from_numpy(drive.astype(np.int64) * self.sim.bstride
+                                   + slot).to(self.sim.device)
+            self._groups[idx] = slot
+            self._slot_idx[slot] = idx
+        else:
+            md = self.max_drive
+            r0 = slot * md
+            rows = self._ent_host[r0:r0 + md]
+            rows[:, 0] = slot
+            rows[:, 1] = self.size
+            rows[:drive.size, 0] = (drive.astype(np.int64) * self.sim.bstride
+                                    + slot)
+            rows[:drive.size, 1] = slot
+            self._entries[2 * r0:2 * (r0 + md)].copy_(
+                torch.from_numpy(rows.view(np.int32).reshape(-1)))
         self._rate_host[slot] = np.float32(rate_hz)
 
     def push_rates(self):
         self._rate.copy_(torch.from_numpy(self._rate_host))
 
+    def defer_clock(self):
+        """Windows launched from now on leave the device clock alone (one
+        tick when the run is closed, not one launch per window)."""
+        if self._clock is None or self.sim.struct.clock_ahead >= (1 << 30):
+            self.close_clock()
+            self._clock = self.sim.deferred_clock()
+            self._clock.__enter__()
+
+    def close_clock(self):
+        run, self._clock = self._clock, None
+        if run is not None:
+            run.__exit__(None, None, None)
+
     def window(self, W):
         s = self.sim
-        L.check(s.lib.fis_drive_table(
-            s._sp, ctypes_ptr(self._entries), self.size * self.max_drive,
-            ctypes_ptr(self._rate), self.size + 1, 1, 1 << 30, W,
-            ctypes_stream()), "fis_drive_table")
+        if not self.fused:
+            L.check(s.lib.fis_drive_table(
+                s._sp, ctypes_ptr(self._entries), self.size * self.max_drive,
+                ctypes_ptr(self._rate), self.size + 1, 1, 1 << 30, W,
+                ctypes_stream()), "fis_drive_table")
         s.step_launch(W)
 
 
 class Server:
     """Slots for SimRequests, held in shards of ``shard`` trials.
 
-    Each shard is its own DeviceSim; a window launches only the shards
-    that hold a live request, so the GPU time of a window follows the live
-    load rather than the capacity.  The default shard of 16 trials (push
-    road, one launch per window) measured best on the full connectome:
-    p50 33 ms and p99 73 ms at 200 requests/s, against 60 / 155 ms with 32
-    and 115 / 266 ms with 64 (pull road).
+    Each shard is its own DeviceSim with its own HIP stream (default
+    priority), so the windows of different shards overlap on the GPU; a
+    window launches only the shards that hold a live request, so the GPU
+    time of a window follows the live load rather than the capacity.  The
+    lowest slots are ``small`` shards (by default, on the push road, four
+    each of one, four and eight slots): the scheduler fills the lowest free
+    slots first, so at low load requests run nearly as fast as alone (a
+    20-step push window of one trial takes 22 us, of 16 trials 94 us), and
+    the shards of ``shard`` slots take the load beyond them.  Measured on
+    the full connectome (p50 / p99 ms at 200 and 20 requests/s): no small
+    shards 36.0 / 74.9 and 34.9 / 86.2; four of one and four of four 15.5 /
+    56.9 and 7.6 / 19.8; with four of eight as well 14.6 / 49.0 and 7.7 /
+    20.0.  The shard of 16 trials (push road, one launch per window)
+    measured best among uniform shards: p50 33 ms and p99 73 ms at 200
+    requests/s, against 60 / 155 ms with 32 and 115 / 266 ms with 64 (pull
+    road).
 
     ``cancel``, ``evict`` and ``resume`` may be called from any thread (or
     from the ``on_window`` / ``on_result`` callbacks of :meth:`serve`);
-    they take effect at the next window boundary of the serving loop.
+    they take effect at the next window boundary the serving loop reaches
+    (at most ``batch_windows`` windows later).
     """
 
     def __init__(self, net: DeviceNet, slots=256, window=20, road="auto",
                  hbm_fraction=0.5, max_drive=None, graph_windows=0,
-                 shard=16):
+                 shard=16, batch_windows=4, small=None):
         self.net = net
         cap = self.capacity(net, road, hbm_fraction)
         self.slots = int(min(slots, cap))
@@ -319,15 +376,34 @@ class Server:
             raise ValueError("shard must hold at least one slot")
         self.road = pick_road(self.shard, road)
         self.max_drive = int(max_drive or 256)
-        sizes = [self.shard] * (self.slots // self.shard)
-        if self.slots % self.shard:
-            sizes.append(self.slots % self.shard)
+        # the small shards first (the scheduler fills the lowest free slots,
+        # so a lightly loaded server runs its requests in them), then shards
+        # of ``shard`` slots.  By default four each of one, four and eight
+        # slots on the push road; none on the pull road, where every shard
+        # pads to 64 trials
+        if small is None:
+            small = ((1,) * 4 + (4,) * 4 + (8,) * 4 if self.road == "push"
+                     else ())
+        sizes, left = [], self.slots
+        for k in small:
+            if 0 < int(k) <= left:
+                sizes.append(int(k))
+                left -= int(k)
+        sizes += [self.shard] * (left // self.shard)
+        if left % self.shard:
+            sizes.append(left % self.shard)
         self.shards = [_Shard(net, k, pick_road(k, road), self.max_drive)
                        for k in sizes]
+        # slot -> (shard, slot in the shard)
+        self._shard_of = np.repeat(np.arange(len(sizes)), sizes)
+        self._local_of = np.concatenate([np.arange(k) for k in sizes])
         self.sched = Scheduler(self.slots)
         self.t = 0  # the server's clock: steps run since it started
         self._graph = None
         self.graph_windows = int(graph_windows)
+        # windows launched per loop pass when no request can end inside
+        # them and no command waits (each pass schedules once)
+        self.batch_windows = max(1, int(batch_windows))
         self._commands = deque()  # (what, rid), applied between windows
         self._lock = threading.Lock()
         # evicted requests: rid -> (request, trial state, admitted_s)
@@ -343,7 +419,8 @@ class Server:
         return int(free * hbm_fraction // per)
 
     def _where(self, slot):
-        return self.shards[slot // self.shard], slot % self.shard
+        return (self.shards[int(self._shard_of[slot])],
+                int(self._local_of[slot]))
 
     # ------------------------------------------------------------ control
     def cancel(self, rid):
@@ -381,7 +458,8 @@ class Server:
                 if slot is not None:
                     req, t0, adm = self.sched.release(slot)
                     sh, local = self._where(slot)
-                    counts = sh.sim.trial_counts(local)
+                    with sh.on():
+                        counts = sh.sim.trial_counts(local)
                     self._emit(SimResult(
                         req.rid, counts, req.steps, req.arrival_s, adm,
                         time.perf_counter() - self._start, slot,
@@ -410,7 +488,9 @@ class Server:
             elif what == "evict" and slot is not None:
                 req, t0, adm = self.sched.release(slot)
                 sh, local = self._where(slot)
-                state = sh.sim.trial_state(local)
+                with sh.on():
+                    state = sh.sim.trial_state(local)
+                sh.stream.synchronize()  # any shard's stream may load it
                 self.suspended[req.rid] = (req, state, adm)
                 self._retire(slot)
 
@@ -424,6 +504,8 @@ class Server:
         drive tables, the per-trial clocks and the step counters are read
         from device memory, so the graph stays valid while requests come
         and go."""
+        self._close_clocks()
+        self._join_streams()
         g = torch.cuda.CUDAGraph()
         t0 = [sh.sim.t for sh in self.shards]  # capture only records
         try:
@@ -439,30 +521,57 @@ class Server:
                 sh.sim.t = t
         self._graph = g
 
-    def _advance(self):
-        """Run one window of every shard holding a live request -- or, when
-        no slot is free (nothing could be admitted meanwhile), no request
-        ends inside it, no control command is waiting and no shard still
-        consumes a resumed trial's input from its ring, one whole captured
-        graph of windows."""
-        per = self.graph_windows * self.window
-        if (per and not self.sched.free and not self._commands
-                and all(sh.sim.t >= sh.sim.ring_until for sh in self.shards)):
-            left = min(t0 + req.steps for req, t0, _ in
-                       self.sched.active.values()) - self.t
-            if left >= per:
-                if self._graph is None:
-                    self._capture()
-                self._graph.replay()
-                for sh in self.shards:
-                    sh.sim.t += per
-                self.t += per
-                return self.graph_windows
+    def _close_clocks(self):
         for sh in self.shards:
+            with sh.on():
+                sh.close_clock()
+
+    def _join_streams(self):
+        """The current stream waits for every shard's work."""
+        cur = torch.cuda.current_stream()
+        for sh in self.shards:
+            cur.wait_stream(sh.stream)
+
+    def _fork_streams(self):
+        """Every shard's stream waits for the current stream's work."""
+        cur = torch.cuda.current_stream()
+        for sh in self.shards:
+            sh.stream.wait_stream(cur)
+
+    def _advance(self):
+        """Run windows of every shard holding a live request: one, or up to
+        ``batch_windows`` when no request ends inside them and no control
+        command is waiting -- or, when in addition no slot is free (nothing
+        could be admitted meanwhile) and no shard still consumes a resumed
+        trial's input from its ring, one whole captured graph of windows.
+        Eager windows run in each shard's open deferred-clock run."""
+        per = self.graph_windows * self.window
+        left = min(t0 + req.steps for req, t0, _ in
+                   self.sched.active.values()) - self.t
+        if (per and not self.sched.free and not self._commands
+                and all(sh.sim.t >= sh.sim.ring_until for sh in self.shards)
+                and left >= per):
+            if self._graph is None:
+                self._capture()
+            self._close_clocks()
+            self._join_streams()
+            self._graph.replay()
+            self._fork_streams()
+            for sh in self.shards:
+                sh.sim.t += per
+            self.t += per
+            return self.graph_windows
+        k = 1
+        if not self._commands:
+            k = max(1, min(self.batch_windows, left // self.window))
+        for sh in self.shards:  # (a shard's own clock moves with its windows)
             if sh.live:
-                sh.window(self.window)
-        self.t += self.window
-        return 1
+                with sh.on():
+                    sh.defer_clock()
+                    for _ in range(k):
+                        sh.window(self.window)
+        self.t += k * self.window
+        return k
 
     # ------------------------------------------------------------ slots
     def _admit(self, admits):
@@ -476,37 +585,45 @@ class Server:
             by_shard.setdefault(id(sh), (sh, []))[1].append(
                 (local, req, drv, done))
         for sh, items in by_shard.values():
-            s = sh.sim
-            fresh = [(lo, r, d) for lo, r, d, done in items if done == 0]
-            for local, req, drv, done in items:
-                sh.set_slot_drive(local, drv, req.rate_hz)
-                if done:  # a resumed request: its saved column goes back
-                    _, state, _ = self.suspended.pop(req.rid)
-                    s.load_trial_state(local, state)
-            if fresh:
-                out_ne, out_tr, free_ne, free_tr = [], [], [], []
-                for local, req, drv in fresh:
-                    sil = np.unique(np.asarray(req.silenced, np.int64))
-                    out_ne.append(sil)
-                    out_tr.append(np.full(sil.size, local))
-                    free_ne.append(drv)
-                    free_tr.append(np.full(drv.size, local))
-                s.restart_trials([lo for lo, _, _ in fresh],
-                                 [int(r.seed) for _, r, _ in fresh],
-                                 [s.t + int(r.steps) for _, r, _ in fresh],
-                                 out_neurons=np.concatenate(out_ne),
-                                 out_trials=np.concatenate(out_tr),
-                                 free_neurons=np.concatenate(free_ne),
-                                 free_trials=np.concatenate(free_tr))
-            sh.push_rates()
-            sh.live += len(items)
+            with sh.on():
+                self._admit_shard(sh, items)
+
+    def _admit_shard(self, sh, items):
+        s = sh.sim
+        fresh = [(lo, r, d) for lo, r, d, done in items if done == 0]
+        for local, req, drv, done in items:
+            sh.set_slot_drive(local, drv, req.rate_hz)
+            if done:  # a resumed request: its saved column goes back
+                _, state, _ = self.suspended.pop(req.rid)
+                s.load_trial_state(local, state)
+                # (the saved tensors may belong to another shard's stream:
+                # they are not freed before this stream has read them)
+                sh.stream.synchronize()
+        if fresh:
+            out_ne, out_tr, free_ne, free_tr = [], [], [], []
+            for local, req, drv in fresh:
+                sil = np.unique(np.asarray(req.silenced, np.int64))
+                out_ne.append(sil)
+                out_tr.append(np.full(sil.size, local))
+                free_ne.append(drv)
+                free_tr.append(np.full(drv.size, local))
+            s.restart_trials([lo for lo, _, _ in fresh],
+                             [int(r.seed) for _, r, _ in fresh],
+                             [s.t + int(r.steps) for _, r, _ in fresh],
+                             out_neurons=np.concatenate(out_ne),
+                             out_trials=np.concatenate(out_tr),
+                             free_neurons=np.concatenate(free_ne),
+                             free_trials=np.concatenate(free_tr))
+        sh.push_rates()
+        sh.live += len(items)
 
     def _retire(self, slot):
         sh, local = self._where(slot)
-        sh._rate_host[local] = 0.0
-        sh.sim.set_trial_end([local], [0])
-        sh.live -= 1
-        sh.push_rates()
+        with sh.on():
+            sh._rate_host[local] = 0.0
+            sh.sim.set_trial_end([local], [0])
+            sh.live -= 1
+            sh.push_rates()
 
     # ------------------------------------------------------------ loop
     def serve(self, requests, realtime=True, on_result=None, on_window=None):
@@ -552,11 +669,13 @@ class Server:
             busy_windows += self._advance()
             for slot, req, t0, adm in self.sched.finished(self.t):
                 sh, local = self._where(slot)
-                counts = sh.sim.trial_counts(local)  # synchronises
+                with sh.on():
+                    counts = sh.sim.trial_counts(local)  # synchronises
                 self._emit(SimResult(req.rid, counts, req.steps,
                                      req.arrival_s, adm,
                                      time.perf_counter() - start, slot))
                 self._retire(slot)
+        self._close_clocks()
         torch.cuda.synchronize()
         wall = time.perf_counter() - start
         self._on_result = None

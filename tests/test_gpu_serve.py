@@ -35,18 +35,29 @@ def net():
     return DeviceNet(conn)
 
 
-@pytest.mark.parametrize("road,slots,count,shard", [("push", 6, 24, 64),
-                                                    ("pull", 6, 24, 64),
-                                                    ("pull", 70, 24, 64),
-                                                    ("pull", 128, 150, 64),
-                                                    ("push", 40, 60, 16)])
+@pytest.mark.parametrize("road,slots,count,shard,small",
+                         [("push", 6, 24, 64, ()),
+                          ("pull", 6, 24, 64, None),
+                          ("pull", 70, 24, 64, None),
+                          ("pull", 128, 150, 64, None),
+                          ("push", 40, 60, 16, ()),
+                          ("push", 40, 60, 16, None),
+                          ("push", 9, 40, 16, (1, 2, 2))])
 def test_result_is_the_same_alone_and_in_the_mix(net, road, slots, count,
-                                                 shard):
+                                                 shard, small):
     """More requests than slots: slots (and whole shards, launched only
-    while they hold a live request) are freed and reused mid-run."""
+    while they hold a live request, each on its own stream) are freed and
+    reused mid-run; small shards take the lowest slots."""
     reqs = requests(net.n, count, seed=slots)
-    srv = Server(net, slots=slots, road=road, shard=shard)
-    assert len(srv.shards) == -(-slots // min(shard, slots))
+    srv = Server(net, slots=slots, road=road, shard=shard, small=small)
+    sizes = [sh.size for sh in srv.shards]
+    assert sum(sizes) == slots and max(sizes) <= shard
+    if small is None and road == "push":
+        assert sizes[:8] == [1, 1, 1, 1, 4, 4, 4, 4]  # (then 8s, as fit)
+    elif small:
+        assert sizes[:len(small)] == list(small)
+    else:
+        assert len(sizes) == -(-slots // min(shard, slots))
     mix, stats = srv.serve([SimRequest(**vars(r)) for r in reqs],
                            realtime=False)
     assert stats["requests"] == len(reqs)
