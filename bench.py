@@ -26,7 +26,8 @@ over the launch's average duration from HIP events on its stream;
 ``traffic`` is the HBM bytes per launch rocprofv3 counted (FETCH_SIZE and
 WRITE_SIZE, committed under profiles/); ``valu_issue_frac`` is the
 profiled VALU instructions per launch over what the chip issues in that
-time.  ``cpu_baseline`` times the oracle's torch restatement of the
+time.  ``b1.roofline`` puts the plain B = 1 kernel against the same peak,
+by its per-window byte model and by the scope table's per-step formula.  ``cpu_baseline`` times the oracle's torch restatement of the
 reference step (csrmv via torch.sparse.mm) on this host's cores.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
@@ -447,6 +448,32 @@ def summarize(paths):
         print()
 
 
+def b1_roofline(rr, conn):
+    """The plain B = 1 kernel against the HBM roofline, two ways: the bytes
+    one window must move (``push_window_bytes``: state once per window, the
+    ring slots it reads, the scatter) over the launch's median time, and
+    the scope table's per-step formula (26 B per neuron-step priced at every
+    step, plus the scatter) over the measured time per step -- the
+    definition the plain kernel's 40 % target is stated in."""
+    r = rr["runner"]
+    W = r.window
+    launch = rr["kt"]["update_scatter"]
+    wb, parts, extra = window_model(r, conn)
+    spts = rr["spikes"] / (rr["wl"].batch * rr["steps"])
+    st = scope_table_bytes(conn, conn.n, 1, W, spts) / W  # per step
+    step_s = rr["wall"] / rr["steps"]
+    achieved = wb / (launch * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": "k_update<20, true>",
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "launch_ms": round(launch, 5), "window_bytes": int(wb),
+            "window_bytes_parts": parts,
+            "achieved": round(achieved, 1),
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "scope_table_bytes_per_step": int(st),
+            "scope_table_gbs": round(st / step_s / 1e9, 1),
+            "scope_table_frac": round(st / step_s / 1e9 / HBM_PEAK_GBS, 4)}
+
+
 def _short(name):
     name = name.replace("(anonymous namespace)::", "")
     return name[:60]
@@ -662,6 +689,8 @@ def main():
     for name, rr in res.items():
         if rr is not head:
             out[name] = leg_summary(rr)
+    if "b1" in res and res["b1"] is not head and not args.no_model:
+        out["b1"]["roofline"] = b1_roofline(res["b1"], conn)
     out["spikes_per_trial_step"] = {
         name: round(rr["spikes"] / (rr["wl"].batch * rr["steps"]), 3)
         for name, rr in res.items()}
