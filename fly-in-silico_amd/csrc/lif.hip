@@ -1446,14 +1446,14 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
   };
   // the lane's first in-edge, then its act word
   auto first_edge = [&](Tile& T) {
-    const int e = T.off[0] + grp * kWave + lane;
+    const int e = T.off[0] + lane * NG + grp;
     if (T.gather && e < T.e1) {
       T.src = s.in_cols[e];
       T.val = s.in_vals[e];
     }
   };
   auto first_act = [&](Tile& T) {
-    const int e = T.off[0] + grp * kWave + lane;
+    const int e = T.off[0] + lane * NG + grp;
     if (T.gather && e < T.e1)
       T.a = s.act[T.src * parts + T.part];
   };
@@ -1484,13 +1484,15 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
     // handful), not as long as the densest of 20 steps of the in-edge.
     if (T.gather) {
       const int e1 = T.e1;
-      int e = T.off[0] + grp * kWave + lane;  // this lane's in-edge
+      // this lane's in-edge: the groups take the tile's in-edges in turn,
+      // lane by lane, so that they get as many as one another
+      int e = T.off[0] + lane * NG + grp;
       int src = T.src;
       int32_t val = T.val;
       uint64_t a = T.a;
       const int rot = (kRec - rbase) % kRec;
       // uniform over the wave: its first in-edge is below e1
-      while (e - lane < e1) {
+      while (e - lane * NG < e1) {
         const int en = e + kEdgeStep;
         int src_n = 0;
         int32_t val_n = 0;
