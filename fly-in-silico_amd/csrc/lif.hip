@@ -1468,9 +1468,13 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
     const bool act = pl < np && (!s.word_end || t0 < s.word_end[tb >> 6]);
 
 #pragma unroll
+    // the accumulators cleared and the waves met while the first in-edges
+    // are on their way (the barrier's wait for the slowest wave hides in
+    // that latency); then their act words
     for (int j = 0; j < WMAX; j += 4)
       *(int4*)(acc + tid * WMAX + j) = make_int4(0, 0, 0, 0);
     __syncthreads();
+    first_act(T);
 
     // 1. gather.  The tile's in-edges are read 64 at a time, one per lane,
     // with their presynaptic act words (the next 64 are fetched meanwhile;
@@ -1835,7 +1839,6 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
   Tile T = head(tile);
   load(T);
   first_edge(T);
-  first_act(T);
   run(T);
 }
 
