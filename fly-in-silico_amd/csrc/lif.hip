@@ -1632,6 +1632,16 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
     }
     __syncthreads();
 
+    // From here on the arguments are read again from the kernel-argument
+    // segment (scalar loads the compiler cannot merge with the earlier
+    // ones): held across the gather instead, they overflow the scalar
+    // registers into vector-register lanes, and every spill and reload is
+    // one more vector instruction.  (`s` is the kernel argument in both
+    // callers, the first in the segment.)
+    uint64_t ka = (uint64_t)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(ka));
+    const fis_sim& s = *(const fis_sim*)kernarg_ptr(ka);
+
     // 2. update
     float v = T.v, g = T.g, vh = 0.0f;
     uint32_t st = T.st, fl = T.fl, sb = T.sb;
