@@ -732,6 +732,25 @@ __device__ __forceinline__ void scatter_row_steps(const fis_sim& s,
   }
 }
 
+// A kernel-argument address (as an integer) as a pointer into the
+// constant kernel-argument segment, so that loads through it are scalar.
+__device__ __forceinline__ const __attribute__((address_space(4))) void*
+kernarg_ptr(uint64_t a) {
+  return (const __attribute__((address_space(4))) void*)a;
+}
+
+// The kernel's fis_sim argument (the first in the kernel-argument segment)
+// read again, through an address the compiler cannot see through: its
+// fields are then scalar loads at their use, not values kept from the
+// kernel's start.  Kept across a long kernel instead, they overflow the
+// scalar registers into vector-register lanes, and every spill and reload
+// is one more vector instruction.
+__device__ __forceinline__ const fis_sim& args_again() {
+  uint64_t ka = (uint64_t)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(ka));
+  return *(const fis_sim*)kernarg_ptr(ka);
+}
+
 // W steps per neuron-trial.  WMAX is the unrolled bound (1 or 20).
 // Index arithmetic is 32-bit: n * bstride < 2^31 is checked on the host.
 template <int WMAX, bool FUSED>
@@ -770,6 +789,9 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
       sl = (sl + 1 == FIS_RING_LEN) ? 0 : sl + 1;
     }
   }
+  // the step masks k < W are formed again where used (scalar compares),
+  // not kept from the loads above in scalar registers that would spill
+  asm volatile("" : "+s"(W));
   float v = 0.0f, g = 0.0f, vh = 0.0f;
   uint32_t st = 0, fl = FIS_FIRE_OFF, sb = 0;
   if (act) load_cell(s, idx, v, g, st, fl, sb, vh);
@@ -819,6 +841,9 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
   const int kstop = left <= 0 ? 0 : (left >= W ? W : (int)left);
   const uint32_t nspk = __popc(fired_bits & ((1u << kstop) - 1u));
 
+  // the stores and spikes out read the arguments again
+  {
+  const fis_sim& s = args_again();
   if constexpr (FUSED) {
     // below 64 trials the wave adds its own spikes into the ring slots no
     // step of this window reads (no event records, no scatter launch)
@@ -886,7 +911,7 @@ __global__ __launch_bounds__(kBlock) void k_update(fis_sim s, int W) {
   }
   if (bs < kWave && lane == 0 && act)
     s.ev_count[(int64_t)blockIdx.x * FIS_LINE_INTS + (wave & 3)] = nev;
-
+  }
 }
 
 // ------------------------------------------------------------- scatter
@@ -1296,17 +1321,6 @@ __global__ __launch_bounds__(kBlock) void k_pull_plan(fis_sim s, int W,
 #endif
 constexpr int kPullListWaves = FIS_PULL_LIST_WAVES;
 
-// Scalar registers capped at 80: measured on MI355X, a workgroup of this
-// kernel is resident one fewer per CU than the compiler's occupancy
-// estimate whenever the kernel uses more than ~80 (96 SGPRs: 7 of 8
-// workgroups, 1.465 ms per window; capped at 80: 8, 1.417 ms).
-// A kernel-argument address (as an integer) as a pointer into the
-// constant kernel-argument segment, so that loads through it are scalar.
-__device__ __forceinline__ const __attribute__((address_space(4))) void*
-kernarg_ptr(uint64_t a) {
-  return (const __attribute__((address_space(4))) void*)a;
-}
-
 // One tile's window of k_pull (below): `acc` is the workgroup's LDS.
 template <int WMAX, int BS, int TILE, bool GATE>
 __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
@@ -1632,15 +1646,9 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
     }
     __syncthreads();
 
-    // From here on the arguments are read again from the kernel-argument
-    // segment (scalar loads the compiler cannot merge with the earlier
-    // ones): held across the gather instead, they overflow the scalar
-    // registers into vector-register lanes, and every spill and reload is
-    // one more vector instruction.  (`s` is the kernel argument in both
-    // callers, the first in the segment.)
-    uint64_t ka = (uint64_t)__builtin_amdgcn_kernarg_segment_ptr();
-    asm volatile("" : "+s"(ka));
-    const fis_sim& s = *(const fis_sim*)kernarg_ptr(ka);
+    // from here on the arguments are read again (`s` is the kernel argument
+    // in both callers)
+    const fis_sim& s = args_again();
 
     // 2. update
     float v = T.v, g = T.g, vh = 0.0f;
@@ -1852,6 +1860,10 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
   run(T);
 }
 
+// Scalar registers capped at 80: measured on MI355X, a workgroup of this
+// kernel is resident one fewer per CU than the compiler's occupancy
+// estimate whenever the kernel uses more than ~80 (96 SGPRs: 7 of 8
+// workgroups, 1.465 ms per window; capped at 80: 8, 1.417 ms).
 template <int WMAX, int BS, int TILE, bool GATE>
 __global__ __launch_bounds__(TILE) __attribute__((amdgpu_num_sgpr(80)))
 __attribute__((amdgpu_waves_per_eu(8, 8)))
