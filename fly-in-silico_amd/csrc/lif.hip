@@ -594,7 +594,9 @@ __device__ __forceinline__ uint32_t run_window_plain(const fis_sim& s, Cell& c,
 // Returns true, with the cell advanced, when no lane of the wave fired;
 // false, with the cell untouched, when one did (the caller then runs
 // run_window_plain from the same state).  Equal in every value to
-// run_window_plain where it returns true.
+// run_window_plain where it returns true.  (The same speculation for waves
+// with a gate still closed, the gate kept, measured slower: those waves
+// belong to neurons spiking now, and most fire again in the window.)
 template <int WMAX>
 __device__ __forceinline__ bool run_window_open(const fis_sim& s, Cell& c,
                                                 const int32_t (&in)[WMAX]) {
@@ -1731,12 +1733,21 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
     // skip it (quiet_cell).
     const bool plain =
         W == WMAX && !any_lane(!act || fl != 0u || sb != 0u || kstop != W);
-    const bool quiet = plain && !any_lane(!quiet_cell(s, c, in));
+    // (a nonzero conductance or a spike anywhere in the wave rules quiet
+    // out before the 20 inputs are looked at)
+    const bool quiet =
+        plain && !any_lane(c.sp || __float_as_uint(c.g) != 0u) &&
+        !any_lane(!quiet_cell(s, c, in));
     // of the rest, waves with every input gate open try the window without
     // gate and reset first (most waves: most neurons spike in no trial)
     const bool open =
         plain && !quiet &&
         !any_lane(c.sp || c.r + 1 < (int)s.refrac_steps);
+    // waves with a flag but neither fire-off nor hold (driven neurons
+    // without refractoriness, silenced outputs): the kick recurrence
+    const bool kick =
+        !plain && act && W == WMAX &&
+        !any_lane((fl & (FIS_FIRE_OFF | FIS_HOLD)) != 0u);
     uint32_t fired_bits = 0u;
     if (quiet)
       c.r += WMAX;  // the counter's WMAX increments; v, g and sp stay
@@ -1744,6 +1755,8 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
       ;  // no lane fired: fired_bits stay 0
     else if (plain)
       fired_bits = run_window_plain<WMAX>(s, c, in);
+    else if (kick)
+      fired_bits = run_window_kick<WMAX>(s, c, in, sb, stim);
     else if (act)
       fired_bits = run_window<WMAX>(s, c, in, sb, stim, W);
     const uint32_t nspk = __popc(fired_bits & ((1u << kstop) - 1u));
