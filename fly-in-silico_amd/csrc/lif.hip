@@ -1132,6 +1132,18 @@ __device__ __forceinline__ int wave_max_scan(int x) {
   return x;
 }
 
+// Inclusive sum-scan over the 64 lanes of a wave (the same DPP steps as
+// wave_max_scan; lanes outside the shifted rows add zero).
+__device__ __forceinline__ int wave_add_scan(int x) {
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, true);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, true);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, true);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, true);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);
+  return x;
+}
+
 // The owner of pair `base + lane` (base uniform): the last lane L with
 // excl(L) <= base + lane.  The owner of pair `base` is the last lane whose
 // run starts at or before it (a ballot); every lane with pairs whose run
@@ -1192,6 +1204,13 @@ constexpr int kPullShare = FIS_PULL_SHARE;
 #define FIS_PULL_ROUNDS 1
 #endif
 constexpr int kRounds = FIS_PULL_ROUNDS;
+// ... for tiles that read the record lists (parts of 128 and 256 trials),
+// where a round waits for its lists: 2 rounds 32.7 us per step against
+// 33.3 with 1 (256 trials, sensory drive, graph-replayed windows)
+#ifndef FIS_PULL_ROUNDS_LISTS
+#define FIS_PULL_ROUNDS_LISTS 2
+#endif
+constexpr int kRoundsLists = FIS_PULL_ROUNDS_LISTS;
 
 // Neuron-trials per workgroup (fis_sim.pull_tile; 0 = FIS_PULL_TILE).  256
 // lanes (4 waves, 20 KB of accumulators) measured fastest at 256 trials:
@@ -1370,6 +1389,7 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
   // parts of 128 and 256 trials read the record lists (trial numbers fit a
   // byte; below 128 trials a list is no shorter than the one word)
   constexpr bool LISTS = wpt >= 2 && tpn <= 256;
+  constexpr int NR = LISTS ? kRoundsLists : kRounds;  // rounds in flight
   const bool lists = LISTS && s.rec_ids != nullptr;
   const int lane = tid & (kWave - 1);
   const int pl = tid / tpn;                     // position in the tile
@@ -1559,36 +1579,28 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
         const uint32_t m = (e < e1 && live_words)
                                ? (uint32_t)(rot40(a, rot) & wmask) : 0u;
         const uint64_t a_n = en < e1 ? s.act[src_n * parts + part] : 0ull;
-        // exclusive prefix of the lanes' active steps, and their total, bit
-        // by bit of the counts (<= 20): a ballot per bit, counted below
-        // each lane (no round trip through LDS)
+        // exclusive prefix of the lanes' active steps, and their total (a
+        // DPP sum-scan; no round trip through LDS)
         const int pc = __popc(m);
-        int excl = 0, total = 0;
-#pragma unroll
-        for (int b = 0; b < 5; ++b) {
-          const uint64_t mb = __ballot((pc >> b) & 1);
-          excl += (int)__builtin_amdgcn_mbcnt_hi(
-                      (uint32_t)(mb >> 32),
-                      __builtin_amdgcn_mbcnt_lo((uint32_t)mb, 0u))
-                  << b;
-          total += __popcll(mb) << b;
-        }
-        // rounds of 64 pairs, kRounds at a time (their owner searches and
+        const int incl = wave_add_scan(pc);
+        const int excl = incl - pc;
+        const int total = __builtin_amdgcn_readlane(incl, kWave - 1);
+        // rounds of 64 pairs, NR at a time (their owner searches and
         // record loads overlap); shared batches hand the rounds to the
         // waves in turn
         for (int base = mem * kWave; base < total;
-             base += kRounds * kRoundGap) {
-          Pair pr[kRounds];
+             base += NR * kRoundGap) {
+          Pair pr[NR];
 #pragma unroll
-          for (int r = 0; r < kRounds; ++r)
+          for (int r = 0; r < NR; ++r)
             pr[r] = deal_pair(base + r * kRoundGap + lane, excl, m, src, val,
                               pq);
           if constexpr (LISTS) {
             // the step's list of the part: 16 bytes per pair; a lane whose
             // list overflowed (or was marked) reads the record words
-            uint4 li[kRounds];
+            uint4 li[NR];
 #pragma unroll
-            for (int r = 0; r < kRounds; ++r) {
+            for (int r = 0; r < NR; ++r) {
               const bool ok = base + r * kRoundGap + lane < total;
               int sl = rbase + pr[r].step;
               sl = sl >= kRec ? sl - kRec : sl;
@@ -1599,7 +1611,7 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
                                              kRec + sl) * 16);
             }
 #pragma unroll
-            for (int r = 0; r < kRounds; ++r) {
+            for (int r = 0; r < NR; ++r) {
               int32_t* at = acc + pr[r].pos * tpn * WMAX + pr[r].step;
               // the synapse count arrives by a lane shuffle: wait for it
               // here, once, so that the LDS adds below do not each wait for
@@ -1631,9 +1643,9 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
               add_list<WMAX>(at, li[r], fb ? 0u : cnt, cv);
             }
           } else {
-            uint64_t xw[kRounds][wpt];
+            uint64_t xw[NR][wpt];
 #pragma unroll
-            for (int r = 0; r < kRounds; ++r) {
+            for (int r = 0; r < NR; ++r) {
               const bool ok = base + r * kRoundGap + lane < total;
               int sl = rbase + pr[r].step;
               sl = sl >= kRec ? sl - kRec : sl;
@@ -1663,7 +1675,7 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
               }
             }
 #pragma unroll
-            for (int r = 0; r < kRounds; ++r) {
+            for (int r = 0; r < NR; ++r) {
               int32_t* at = acc + pr[r].pos * tpn * WMAX + pr[r].step;
               int32_t cv = pr[r].val;
               asm volatile("s_waitcnt lgkmcnt(0)" : "+v"(cv));
