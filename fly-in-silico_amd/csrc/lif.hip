@@ -1161,8 +1161,28 @@ __device__ __forceinline__ int round_owner(int base, int lane, int excl,
   return wave_max_scan(got) - 1;
 }
 
+// Positions of the set bits of a 5-bit value c, lowest first, 3 bits
+// each: entry c of the table deal_pair reads (lane c holds entry c).
+struct BitPos {
+  uint32_t v[32];
+};
+constexpr BitPos make_bit_pos() {
+  BitPos t{};
+  for (uint32_t c = 0; c < 32; ++c) {
+    uint32_t e = 0, n = 0;
+    for (uint32_t b = 0; b < 5; ++b)
+      if ((c >> b) & 1u) e |= b << (3 * n++);
+    t.v[c] = e;
+  }
+  return t;
+}
+__device__ const BitPos kBitPos = make_bit_pos();
+
+// `bitpos`: this lane's entry of kBitPos (entry lane mod 32).  Masks of at
+// most 20 steps (four 5-bit chunks).
 __device__ __forceinline__ Pair deal_pair(int pi, int excl, uint32_t m,
-                                          int src, int32_t val, int pos) {
+                                          int src, int32_t val, int pos,
+                                          uint32_t bitpos) {
   const int lane = __lane_id();
   const int o = round_owner(pi - lane, lane, excl, m);
   const int eo = __shfl(excl, o);
@@ -1171,19 +1191,21 @@ __device__ __forceinline__ Pair deal_pair(int pi, int excl, uint32_t m,
   p.src = __shfl(src, o);
   p.val = __shfl(val, o);
   p.pos = __shfl(pos, o);
-  // the step: the (pi - eo)-th set bit of the owner's mask
-  int k = pi - eo, j = 0;
-  uint32_t x = mo;
-#pragma unroll
-  for (int wd = 16; wd > 0; wd >>= 1) {
-    const int c = __popc(x & ((1u << wd) - 1u));
-    if (k >= c) {
-      k -= c;
-      x >>= wd;
-      j += wd;
-    }
-  }
-  p.step = j;
+  // the step: the k-th set bit of the owner's mask.  The 5-bit chunk that
+  // holds it by the chunks' running counts, then its place in the chunk
+  // from the table (a lane permute: lane c holds chunk value c's entry).
+  const int k = pi - eo;
+  const int n0 = __popc(mo & 31u);
+  const int n1 = n0 + __popc((mo >> 5) & 31u);
+  const int n2 = n1 + __popc((mo >> 10) & 31u);
+  int below = 0, sh = 0;
+  if (k >= n0) { below = n0; sh = 5; }
+  if (k >= n1) { below = n1; sh = 10; }
+  if (k >= n2) { below = n2; sh = 15; }
+  const uint32_t chunk = (mo >> sh) & 31u;
+  const uint32_t ent =
+      (uint32_t)__builtin_amdgcn_ds_bpermute((int)(chunk << 2), (int)bitpos);
+  p.step = sh + (int)((ent >> (3 * (k - below))) & 7u);
   return p;
 }
 
@@ -1535,6 +1557,10 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
     const uint32_t live_words = T.live;
     // uniform over a wave: a neuron of the tile, in a live 64-trial word
     const bool act = pl < np && (!s.word_end || t0 < s.word_end[tb >> 6]);
+    // the lane's entry of the table deal_pair reads (loaded during the
+    // accumulator clear)
+    static_assert(WMAX <= 20, "deal_pair reads step masks of 20 bits");
+    const uint32_t bitpos = kBitPos.v[lane & 31];
 
 #pragma unroll
     // the accumulators cleared and the waves met while the first in-edges
@@ -1594,7 +1620,7 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
 #pragma unroll
           for (int r = 0; r < NR; ++r)
             pr[r] = deal_pair(base + r * kRoundGap + lane, excl, m, src, val,
-                              pq);
+                              pq, bitpos);
           if constexpr (LISTS) {
             // the step's list of the part: 16 bytes per pair; a lane whose
             // list overflowed (or was marked) reads the record words
@@ -1755,11 +1781,6 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
     const bool open =
         plain && !quiet &&
         !any_lane(c.sp || c.r + 1 < (int)s.refrac_steps);
-    // waves with a flag but neither fire-off nor hold (driven neurons
-    // without refractoriness, silenced outputs): the kick recurrence
-    const bool kick =
-        !plain && act && W == WMAX &&
-        !any_lane((fl & (FIS_FIRE_OFF | FIS_HOLD)) != 0u);
     uint32_t fired_bits = 0u;
     if (quiet)
       c.r += WMAX;  // the counter's WMAX increments; v, g and sp stay
@@ -1767,7 +1788,10 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
       ;  // no lane fired: fired_bits stay 0
     else if (plain)
       fired_bits = run_window_plain<WMAX>(s, c, in);
-    else if (kick)
+    // waves with a flag but neither fire-off nor hold (driven neurons
+    // without refractoriness, silenced outputs): the kick recurrence
+    else if (act && W == WMAX &&
+             !any_lane((fl & (FIS_FIRE_OFF | FIS_HOLD)) != 0u))
       fired_bits = run_window_kick<WMAX>(s, c, in, sb, stim);
     else if (act)
       fired_bits = run_window<WMAX>(s, c, in, sb, stim, W);
