@@ -1411,7 +1411,9 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
   // parts of 128 and 256 trials read the record lists (trial numbers fit a
   // byte; below 128 trials a list is no shorter than the one word)
   constexpr bool LISTS = wpt >= 2 && tpn <= 256;
-  constexpr int NR = LISTS ? kRoundsLists : kRounds;  // rounds in flight
+  // rounds in flight (the gated kernel's work-list loop leaves no
+  // registers for a second round: it would spill)
+  constexpr int NR = LISTS && !GATE ? kRoundsLists : kRounds;
   const bool lists = LISTS && s.rec_ids != nullptr;
   const int lane = tid & (kWave - 1);
   const int pl = tid / tpn;                     // position in the tile
