@@ -241,6 +241,54 @@ def philox_batched(batch, road, fused, tile=None, params=None, gate=False,
     return sim
 
 
+@pytest.mark.parametrize("batch,tile", [(256, 256), (64, 128), (64, 256)])
+def test_windows_from_rest_into_a_burst(batch, tile):
+    """The pull road's recurrence paths against the oracle, each made to
+    run: 60 steps at rest (every input gate open: waves take the window
+    without gate and reset, or none at all where nothing moves), then a
+    tape burst from step 65 into a third of the trials -- the driven waves
+    (no refractoriness, kicks) take the kick recurrence, and waves of their
+    targets that start the window open and fire in it run it again with
+    gate and reset; the next windows hold refractory lanes."""
+    conn = C.synthetic(512, None, mean_out=40, count_exp=1.2, n_groups=4,
+                       seed=5)
+    n, steps = conn.n, 140
+    rng = np.random.default_rng(batch + tile)
+    drv = rng.choice(n, 24, replace=False)
+    trials = np.arange(batch)[rng.random(batch) < 1 / 3]
+    bits = np.zeros((steps, batch, n), np.uint8)
+    burst = rng.random((steps - 65, trials.size, drv.size)) < 0.6
+    bits[65:, trials[:, None], drv[None, :]] = burst
+    t_, i_, b_ = tape_of_bits(bits)
+    sim = DeviceSim(DeviceNet(conn), batch, no_refractory=drv,
+                    raster_len=steps, road="pull", pull_tile=tile)
+    ora = O.OracleEngine(conn, batch, no_refractory=drv)
+    tape = Tape(sim, t_, i_, b_)
+    for a in range(0, steps, 20):
+        sim.window(20, tape)
+        if a == 40:
+            compare_state(sim, ora_run(ora, bits, 0, 60), "rest")
+    raster = sim.read_raster(0, steps)
+    ora_run(ora, bits, 60, steps)
+    ora2 = O.OracleEngine(conn, batch, no_refractory=drv)
+    fired = np.zeros(n, bool)
+    for k in range(steps):
+        ora2.step_bits(bits[k])
+        assert np.array_equal(raster[k], ora2.spikes.astype(bool)), \
+            "raster differs at step %d" % k
+        fired |= ora2.spikes.any(0)
+    compare_state(sim, ora, "end")
+    assert not raster[20:60].any(), "the network was not at rest"
+    undriven = np.setdiff1d(np.arange(n), drv)
+    assert fired[undriven].sum() >= 5, "the burst reached no target"
+
+
+def ora_run(ora, bits, t0, t1):
+    for k in range(t0, t1):
+        ora.step_bits(bits[k])
+    return ora
+
+
 @pytest.mark.parametrize("batch", [64, 100, 200, 600])
 def test_pull_road_uneven_windows_and_out_masks(batch):
     """Pull road with a partial last tile (509 neurons), windows of 1..20
