@@ -499,12 +499,17 @@ def run_leg(name, net, conn, args, wl, steps):
     if gw > 0:
         r.capture(gw)
     use = gw > 0
-    sync = __import__("torch").cuda.synchronize
+    import torch
+    sync = torch.cuda.synchronize
     r.advance(args.preroll, use_graph=use)
     r.advance(args.warmup, use_graph=use)
-    c0 = r.counts().sum()
+    # spike totals summed on the device: a host-side sum of the 35 M
+    # counts leaves the GPU idle for tens of milliseconds right before the
+    # timed steps, and it starts them at lowered clocks
+    total_spikes = lambda: int(r.sim.counts.sum(dtype=torch.int64))  # noqa: E731
+    c0 = total_spikes()
     wall = timed(lambda: r.advance(steps, use_graph=use), sync)
-    spikes = int(r.counts().sum() - c0)
+    spikes = total_spikes() - c0
     kt = kernel_times(r, 20)
     res = dict(wl=wl, runner=r, wall=wall, steps=steps, spikes=spikes,
                kt=kt, tuned=tuned, graph_windows=gw,
