@@ -88,11 +88,17 @@ __device__ __forceinline__ uint32_t window_kicks(int64_t t0, int W,
   const int64_t c0 = t0 >> 2;
   const int off = (int)(t0 & 3);        // word of step t0 in block c0
   const int nc = (off + W + 3) >> 2;    // blocks this window touches
+  // The first NC - 1 blocks unconditionally (a full window needs them all;
+  // a shorter one ignores the surplus words): one basic block, in which
+  // the compiler interleaves their independent rounds.  Each behind a
+  // test of its own, they ran one dependent chain after another.
   U4 x[NC];
 #pragma unroll
-  for (int c = 0; c < NC; ++c)
-    x[c] = c < nc ? philox(U4{(uint32_t)(c0 + c), neuron, tid, 0u}, k0, k1)
-                  : U4{0u, 0u, 0u, 0u};
+  for (int c = 0; c < NC - 1; ++c)
+    x[c] = philox(U4{(uint32_t)(c0 + c), neuron, tid, 0u}, k0, k1);
+  x[NC - 1] = U4{0u, 0u, 0u, 0u};
+  if (nc == NC)  // a window that starts inside a block touches one more
+    x[NC - 1] = philox(U4{(uint32_t)(c0 + NC - 1), neuron, tid, 0u}, k0, k1);
   uint32_t bits = 0;
 #pragma unroll
   for (int i = 0; i < 4 * NC; ++i) {  // word i of the blocks = step i - off
