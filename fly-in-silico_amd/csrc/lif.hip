@@ -1117,8 +1117,8 @@ __device__ __forceinline__ void list_push(uint4& l, uint32_t b) {
 // One active (in-edge, step) pair of a wave's 64 in-edges, dealt to the
 // lane that takes pair number pi: lane L of the 64 owns pairs
 // excl(L) .. excl(L) + popc(m(L)) - 1, one per set step bit of m(L).  All
-// 64 lanes must take part (the owner is found by a binary search over the
-// lanes' excl, read by shuffles).
+// 64 lanes must take part (the owner is found by a forward permute and a
+// DPP max-scan, its fields read by lane shuffles).
 struct Pair {
   int src;      // presynaptic neuron
   int32_t val;  // synapse count
