@@ -1570,12 +1570,16 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
     static_assert(WMAX <= 20, "deal_pair reads step masks of 20 bits");
     const uint32_t bitpos = kBitPos.v[lane & 31];
 
-#pragma unroll
     // the accumulators cleared and the waves met while the first in-edges
     // are on their way (the barrier's wait for the slowest wave hides in
-    // that latency); then their act words
-    for (int j = 0; j < WMAX; j += 4)
-      *(int4*)(acc + tid * WMAX + j) = make_int4(0, 0, 0, 0);
+    // that latency); then their act words.  The zeros are made here (kept
+    // from the kernel's start instead, the work-list loop spilled them to
+    // scratch and read them back for every tile).
+    int z = 0;
+    asm volatile("" : "+v"(z));
+    const int4 z4 = make_int4(z, z, z, z);
+#pragma unroll
+    for (int j = 0; j < WMAX; j += 4) *(int4*)(acc + tid * WMAX + j) = z4;
     __syncthreads();
     first_act(T);
 
@@ -1825,9 +1829,11 @@ __device__ __forceinline__ void pull_tile(const fis_sim& s, int32_t* acc,
       const uint32_t sent = transmit ? fired_bits : 0u;
       int32_t* mine = acc + (tid & ~(kWave - 1)) * WMAX;  // this wave's cells
       uint64_t* cell = (uint64_t*)(mine + 4 * (lane < W ? lane : 0));
-      if (lane < W) {
-        cell[0] = 0ull;
-        cell[1] = 0ull;
+      if (lane < W) {  // (zeros made here, as for the accumulators)
+        uint64_t z = 0ull;
+        asm volatile("" : "+v"(z));
+        cell[0] = z;
+        cell[1] = z;
       }
       if (lane == W) mine[4 * WMAX] = 0;
       if (lane == 0) mine[4 * WMAX + 1] = wave_calm ? 1 : 0;
