@@ -173,11 +173,16 @@ struct TableRate {
   }
 };
 
-// trial b's own step count at global step t (0 at its slot's start)
 // The current step: the clock, plus the steps a deferred run of windows
-// has launched since the clock was last advanced.
+// has launched since the clock was last advanced.  Read as a scalar load
+// (through the constant address space): no kernel that reads the clock
+// writes it (k_tick and the scatter launches advance it and read it not),
+// and a launch sees what earlier launches left.  As a vector load its
+// round trip stood at the start of every tile.
 __device__ __forceinline__ int64_t now(const fis_sim& s) {
-  return *s.clock + s.clock_ahead;
+  return *(const __attribute__((address_space(4))) int64_t*)(uintptr_t)
+             s.clock +
+         s.clock_ahead;
 }
 
 __device__ __forceinline__ int64_t origin_of(const fis_sim& s, int b) {
