@@ -583,9 +583,9 @@ def main():
                          "calibration: known bytes read and written)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the whole batch on "
-              "one MI355X (trials shard across GPUs with no exchange; see "
-              "DESIGN.md).  --gpus %d is not run." % args.gpus,
+        print("bench.py measures one GPU: the whole batch runs on one "
+              "MI355X (trials would shard across GPUs with no exchange).  "
+              "--gpus %d is not run." % args.gpus,
               file=sys.stderr)
         sys.exit(2)
 
