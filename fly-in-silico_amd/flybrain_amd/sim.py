@@ -292,6 +292,9 @@ class Engine:
                               no_refractory=free, spikes_out=True,
                               road="push" if road == "auto" else road)
         self.road = self._sim.road
+        # the [batch, n] view of the spike output, made once: step returns
+        # this same tensor every call, updated in place (sim.py:745-771)
+        self._spikes = self._sim.view(self._sim.spikes_out)
         self._clock_run = None  # an open deferred-clock run of eager steps
         self.gen = _Gen(self, seed)
         self.refrac_steps = torch.full((self.n,), float(round(p["tRefrac"] / DT)),
@@ -316,7 +319,7 @@ class Engine:
 
     @property
     def spikes(self):
-        return self._sim.view(self._sim.spikes_out)
+        return self._spikes
 
     @property
     def refrac(self):
@@ -450,13 +453,13 @@ class Engine:
     # --- stepping -----------------------------------------------------------
     def _launch_step(self, rates):
         self._clock_defer()
-        self._sim.step_rates(rates, 1)
+        self._sim.step_rates(rates, 1, checked=True)
 
     def use_graphs(self, enable=True, force=False):
         """Capture one step (drive + update + scatter) into a HIP graph.
 
         Returns whether steps now replay a graph.  On MI355X a captured
-        step replays slower than its direct launches (B = 1: 34 against 11
+        step replays slower than its direct launches (B = 1: 34 against 6
         us per step), so, as the reference stays ungraphed where graphs do
         not serve (sim.py:676-677, 715-737), this declines and returns False
         unless ``force`` -- results are the same either way.
@@ -513,10 +516,12 @@ class Engine:
         """Advance one 0.1 ms tick.  ``rates`` is [batch, n] in Hz."""
         self.calls += 1
         r = rates
-        if (not torch.is_tensor(r) or r.device != self.device
-                or r.dtype != torch.float32 or not r.is_contiguous()):
+        # (the cheapest tests first: this is the per-step host path)
+        if not (type(r) is torch.Tensor and r.dtype is torch.float32
+                and r.is_cuda and r.get_device() == self.device.index
+                and r.is_contiguous()):
             r = torch.as_tensor(r, dtype=torch.float32).to(self.device).contiguous()
-        if tuple(r.shape) != (self.batch, self.n):
+        if r.shape != (self.batch, self.n):
             raise ValueError("rates must be [%d, %d], got %s"
                              % (self.batch, self.n, tuple(r.shape)))
         if self._graph is not None and self._graph_version != self._sim.version:
