@@ -55,6 +55,37 @@ import torch
 
 from . import _lib as L
 from .device import NO_END, DeviceNet, DeviceSim, check_rates_hz, pick_road
+from .device import _stream
+
+# torch's current-stream slot read and set by stream id: what
+# torch.cuda.stream() does for a stream of the current device, without the
+# Stream and device objects it builds on every entry (~6 us; the serving
+# loop enters a shard's stream once per shard and pass)
+_get_cur = getattr(torch._C, "_cuda_getCurrentStream", None)
+_set_cur = getattr(torch._C, "_cuda_setStream", None)
+
+
+class _OnStream:
+    """torch's current stream on ``stream``'s device set to ``stream`` for
+    the block, the previous one restored after it."""
+
+    __slots__ = ("sid", "dev", "typ", "prev")
+
+    def __init__(self, stream):
+        self.sid = stream.stream_id
+        self.dev = stream.device_index
+        self.typ = stream.device_type
+
+    def __enter__(self):
+        self.prev = _get_cur(self.dev)
+        _set_cur(stream_id=self.sid, device_index=self.dev,
+                 device_type=self.typ)
+        return self
+
+    def __exit__(self, *exc):
+        p = self.prev
+        _set_cur(stream_id=p[0], device_index=p[1], device_type=p[2])
+        return False
 
 
 @dataclass
@@ -284,7 +315,9 @@ class _Shard:
     def on(self):
         """Run on the shard's stream: the work of different shards overlaps
         on the GPU, a shard's own stays in order."""
-        return torch.cuda.stream(self.stream)
+        if _get_cur is None or _set_cur is None:
+            return torch.cuda.stream(self.stream)
+        return _OnStream(self.stream)
 
     def set_slot_drive(self, slot, drive, rate_hz):
         if self.fused:
@@ -330,7 +363,7 @@ class _Shard:
             L.check(s.lib.fis_drive_table(
                 s._sp, ctypes_ptr(self._entries), self.size * self.max_drive,
                 ctypes_ptr(self._rate), self.size + 1, 1, 1 << 30, W,
-                ctypes_stream()), "fis_drive_table")
+                _stream()), "fis_drive_table")
         s.step_launch(W)
 
 
@@ -728,11 +761,6 @@ def poisson_requests(n, count, rate_req_s, seed=0, steps=(500, 20_000),
 def ctypes_ptr(t):
     import ctypes
     return ctypes.c_void_p(t.data_ptr())
-
-
-def ctypes_stream():
-    import ctypes
-    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
 __all__ = ["SimRequest", "SimResult", "Scheduler", "Server", "NO_END",

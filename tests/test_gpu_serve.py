@@ -6,6 +6,7 @@ from __future__ import annotations
 
 import numpy as np
 import pytest
+import torch
 
 import oracle as O
 from flybrain_amd import connectome as C
@@ -183,3 +184,19 @@ def test_bad_requests_cancel_and_eviction_leave_the_rest_unchanged(
         assert res[rid].ok
         np.testing.assert_array_equal(res[rid].counts, counts,
                                       err_msg="request %d" % rid)
+
+
+def test_shard_stream_context_sets_and_restores_the_current_stream(net):
+    """A shard's stream context makes its stream torch's current one (the
+    launches and torch's own work inside go there) and puts the previous
+    one back, also when the block raises."""
+    srv = Server(net, slots=3, road="push", shard=2, small=())
+    before = torch.cuda.current_stream()
+    for sh in srv.shards:
+        with sh.on():
+            assert torch.cuda.current_stream() == sh.stream
+        assert torch.cuda.current_stream() == before
+    with pytest.raises(KeyError):
+        with srv.shards[0].on():
+            raise KeyError("inside")
+    assert torch.cuda.current_stream() == before
