@@ -624,6 +624,20 @@ class DeviceSim:
         col = self.counts.view(self.n, self.bstride)[:, int(trial)]
         return col.to(torch.int64).cpu().numpy()
 
+    def trial_counts_async(self, trial):
+        """Spike counts of one trial copied to pinned host memory on the
+        current stream without waiting: (host int64 [n] tensor, event
+        recorded after the copy).  The counts are there once the event has
+        completed."""
+        if self.counts is None:
+            raise RuntimeError("built without counts")
+        col = self.counts.view(self.n, self.bstride)[:, int(trial)]
+        host = torch.empty(self.n, dtype=torch.int64, pin_memory=True)
+        host.copy_(col.to(torch.int64), non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record()
+        return host, ev
+
     def set_hold(self, neurons, v_clamp, trials=None):
         tr = self._trials(trials)
         ne = self._neurons(neurons)

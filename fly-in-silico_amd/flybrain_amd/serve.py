@@ -527,6 +527,19 @@ class Server:
                 self.suspended[req.rid] = (req, state, adm)
                 self._retire(slot)
 
+    def _land(self, landing, start, wait=False):
+        """Emit the results whose counts have reached the host (all of
+        them with ``wait``, after a synchronisation); the rest stay."""
+        left = []
+        for (host, ev), req, adm, slot in landing:
+            if wait or ev.query():
+                self._emit(SimResult(req.rid, host.numpy(), req.steps,
+                                     req.arrival_s, adm,
+                                     time.perf_counter() - start, slot))
+            else:
+                left.append(((host, ev), req, adm, slot))
+        return left
+
     def _emit(self, res):
         self._results.append(res)
         if self._on_result is not None:
@@ -684,7 +697,10 @@ class Server:
                 r.arrival_s = 0.0
             self.sched.submit(r)
         busy_windows = 0
+        landing = []  # (counts copy in flight, request, admitted, slot)
         while (self.sched.pending or self.sched.busy or self._commands):
+            if landing:  # (also while the loop waits for arrivals)
+                landing = self._land(landing, start)
             now = time.perf_counter() - start
             if on_window is not None:
                 on_window(self)
@@ -700,16 +716,20 @@ class Server:
                     break  # only suspended requests are left
                 continue
             busy_windows += self._advance()
+            # a finished request's counts are copied to the host on its
+            # shard's stream, and its slot freed at once (whatever runs
+            # there next is ordered after the copy); the result is emitted
+            # once the copy has landed, so the loop never waits for it
             for slot, req, t0, adm in self.sched.finished(self.t):
                 sh, local = self._where(slot)
                 with sh.on():
-                    counts = sh.sim.trial_counts(local)  # synchronises
-                self._emit(SimResult(req.rid, counts, req.steps,
-                                     req.arrival_s, adm,
-                                     time.perf_counter() - start, slot))
+                    landing.append((sh.sim.trial_counts_async(local), req,
+                                    adm, slot))
                 self._retire(slot)
+            landing = self._land(landing, start)
         self._close_clocks()
         torch.cuda.synchronize()
+        self._land(landing, start, wait=True)
         wall = time.perf_counter() - start
         self._on_result = None
         return results, summarize(results, wall, self.slots, busy_windows,
