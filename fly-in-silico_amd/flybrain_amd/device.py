@@ -69,14 +69,18 @@ def _ptr(t):
 
 
 # torch's current HIP stream of the current device as a raw handle: the
-# private getter costs ~0.1 us a call, torch.cuda.current_stream() ~2.8 us
-# (most of a one-step Engine.step's host time)
+# private getters cost ~0.1 us a call each, torch.cuda.current_stream()
+# ~2.8 us (most of a one-step Engine.step's host time) and
+# torch.cuda.current_device() ~0.8.  Every launch comes after the
+# simulation's tensors were allocated on the device, so torch's GPU state
+# is initialised by then.
 _raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+_get_device = getattr(torch._C, "_cuda_getDevice", None)
 
 
 def _stream():
-    if _raw_stream is not None:
-        return ctypes.c_void_p(_raw_stream(torch.cuda.current_device()))
+    if _raw_stream is not None and _get_device is not None:
+        return ctypes.c_void_p(_raw_stream(_get_device()))
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
