@@ -459,7 +459,7 @@ class Engine:
         """Capture one step (drive + update + scatter) into a HIP graph.
 
         Returns whether steps now replay a graph.  On MI355X a captured
-        step replays slower than its direct launches (B = 1: 34 against 6
+        step replays slower than its direct launches (B = 1: 22 against 5
         us per step), so, as the reference stays ungraphed where graphs do
         not serve (sim.py:676-677, 715-737), this declines and returns False
         unless ``force`` -- results are the same either way.
